@@ -1,0 +1,45 @@
+/* Substate text IO (OpenCAL-OMP/include/cal2DIO.h:21-71, cal2DBufferIO.h:22-72).
+ * Whitespace-separated matrices; reals are written with "%.9f" as the
+ * OpenMP reference does (OpenCAL-OMP/source/cal2DBufferIO.c:142).  The save
+ * functions first bring the substate back from the GPU when a device run
+ * left it there. */
+#ifndef cal2DIO_h
+#define cal2DIO_h
+
+#include <stdio.h>
+#include <cal2D.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void calfLoadMatrix2Db(CALbyte* M, int rows, int columns, FILE* f);
+void calfLoadMatrix2Di(CALint* M, int rows, int columns, FILE* f);
+void calfLoadMatrix2Dr(CALreal* M, int rows, int columns, FILE* f);
+CALbyte calLoadMatrix2Db(CALbyte* M, int rows, int columns, char* path);
+CALbyte calLoadMatrix2Di(CALint* M, int rows, int columns, char* path);
+CALbyte calLoadMatrix2Dr(CALreal* M, int rows, int columns, char* path);
+void calfSaveMatrix2Db(CALbyte* M, int rows, int columns, FILE* f);
+void calfSaveMatrix2Di(CALint* M, int rows, int columns, FILE* f);
+void calfSaveMatrix2Dr(CALreal* M, int rows, int columns, FILE* f);
+CALbyte calSaveMatrix2Db(CALbyte* M, int rows, int columns, char* path);
+CALbyte calSaveMatrix2Di(CALint* M, int rows, int columns, char* path);
+CALbyte calSaveMatrix2Dr(CALreal* M, int rows, int columns, char* path);
+
+void calfLoadSubstate2Db(struct CALModel2D* ca2D, struct CALSubstate2Db* Q, FILE* f);
+void calfLoadSubstate2Di(struct CALModel2D* ca2D, struct CALSubstate2Di* Q, FILE* f);
+void calfLoadSubstate2Dr(struct CALModel2D* ca2D, struct CALSubstate2Dr* Q, FILE* f);
+CALbyte calLoadSubstate2Db(struct CALModel2D* ca2D, struct CALSubstate2Db* Q, char* path);
+CALbyte calLoadSubstate2Di(struct CALModel2D* ca2D, struct CALSubstate2Di* Q, char* path);
+CALbyte calLoadSubstate2Dr(struct CALModel2D* ca2D, struct CALSubstate2Dr* Q, char* path);
+void calfSaveSubstate2Db(struct CALModel2D* ca2D, struct CALSubstate2Db* Q, FILE* f);
+void calfSaveSubstate2Di(struct CALModel2D* ca2D, struct CALSubstate2Di* Q, FILE* f);
+void calfSaveSubstate2Dr(struct CALModel2D* ca2D, struct CALSubstate2Dr* Q, FILE* f);
+CALbyte calSaveSubstate2Db(struct CALModel2D* ca2D, struct CALSubstate2Db* Q, char* path);
+CALbyte calSaveSubstate2Di(struct CALModel2D* ca2D, struct CALSubstate2Di* Q, char* path);
+CALbyte calSaveSubstate2Dr(struct CALModel2D* ca2D, struct CALSubstate2Dr* Q, char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

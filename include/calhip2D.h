@@ -1,0 +1,118 @@
+/* MI355X engine extensions to the OpenCAL 2D API.
+ *
+ * These entry points have no counterpart in OpenCAL: they attach hand-written
+ * HIP device programs to a model's elementary processes, move state between
+ * the GPU and the host mirrors, report errors, and expose the two additions
+ * that change the method's form (the batched ensemble and the SciddicaT
+ * adjoint).  Plain C ABI: pointers, sizes, enums — no C++ or torch types.
+ */
+#ifndef calhip2D_h
+#define calhip2D_h
+
+#include <stddef.h>
+#include <cal2D.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+
+/* 0 when no error happened since the last calhipClearError(); otherwise a
+ * non-zero code (the first error is kept) and a message. */
+int calhipGetLastError(void);
+const char* calhipGetErrorString(void);
+void calhipClearError(void);
+
+/* --------------------------------------------------------------- residency */
+
+/* Bring every substate and the active set of the model back into the host
+ * mirrors (Q->current / Q->next, A.flags, A.cells, A.size_current).  Needed
+ * only by code that reads those fields directly while a device run is in
+ * progress; the accessor and IO functions do it themselves. */
+int calhipSyncHost2D(struct CALModel2D* ca2D);
+
+/* Wait for the model's queued GPU work. */
+int calhipSynchronize2D(struct CALModel2D* ca2D);
+
+/* 1 if the model's state currently lives on the GPU. */
+int calhipIsOnDevice2D(struct CALModel2D* ca2D);
+
+/* Counters since model creation. */
+struct calhipStats {
+	long long steps;           /* device program steps */
+	long long launches;        /* kernel launches */
+	long long uploads;         /* host -> device model transfers */
+	long long downloads;       /* device -> host model transfers */
+	long long rebuilds;        /* active-set compactions */
+	long long materialisations;/* transient substates written out on demand */
+};
+int calhipGetStats2D(struct CALModel2D* ca2D, struct calhipStats* out);
+
+/* Kernel timing with HIP events recorded on the model's own stream around
+ * each launch of the program's main sweep kernel.  enable=1 starts, 0 stops;
+ * calhipGetKernelTime2D returns the accumulated milliseconds and launches. */
+int calhipEnableKernelTiming2D(struct CALModel2D* ca2D, int enable);
+int calhipGetKernelTime2D(struct CALModel2D* ca2D, double* total_ms, long long* launches);
+
+/* ------------------------------------------------------- device programs */
+
+/* Flow substates of SciddicaT are transient when the model's steering resets
+ * them to zero every step (as OpenCAL's sciddicaT.c:133-146 does): the fused
+ * program then never writes them to memory.  A steering that does not reset
+ * them, or any read of them between steps, makes the program write them out
+ * on demand. */
+
+/* Attach the SciddicaT device program to a model built as the reference
+ * example builds it (sciddicaT.c:143-184): von Neumann, toroidal, substates
+ * z, h, f[4], processes `flows`, `width` and (active-cells variant) `remove`
+ * registered in that order.  p_r / p_epsilon point to the model's parameter
+ * variables and are read at every step.  remove may be NULL for CAL_NO_OPT. */
+int calhipBindSciddicaT2D(struct CALModel2D* ca2D,
+                          CALCallbackFunc2D flows, CALCallbackFunc2D width, CALCallbackFunc2D remove,
+                          struct CALSubstate2Dr* z, struct CALSubstate2Dr* h,
+                          struct CALSubstate2Dr* f[4],
+                          const CALreal* p_r, const CALreal* p_epsilon);
+
+/* Attach the Life device program (one process, Moore, toroidal) to a byte or
+ * int substate.  Exactly one of Qb / Qi is non-NULL. */
+int calhipBindLife2D(struct CALModel2D* ca2D, CALCallbackFunc2D transition,
+                     struct CALSubstate2Db* Qb, struct CALSubstate2Di* Qi);
+
+/* --------------------------------------------------- batched SciddicaT */
+
+/* Run `steps` full-grid SciddicaT steps on `n` independent instances of
+ * uneven size and parameters in one persistent launch.
+ *   rows[k], cols[k]   grid of instance k (each >= 3)
+ *   p_r[k], p_eps[k]   parameters of instance k
+ *   z_host[k], h_host[k]  host arrays (rows[k]*cols[k]) with the state AFTER
+ *                      the reference init (z already lowered by h)
+ *   h_out[k]           host arrays receiving the final thickness
+ * timing_ms (optional): GPU time of the stepping launch(es) only. */
+int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols,
+                            const double* p_r, const double* p_eps,
+                            const double* const* z_host, const double* const* h_host,
+                            int steps, double* const* h_out, double* timing_ms);
+
+/* --------------------------------------------------- SciddicaT adjoint */
+
+/* Device-pointer entry points (pointers from hipMalloc or a framework's
+ * device allocator), used by the torch.autograd.Function wrapper.
+ * All grids are rows x cols row-major doubles, toroidal.
+ *
+ * Forward: h_out = S^steps(h0) with checkpoints written every `ck` steps into
+ * ckpt (capacity (steps/ck + 1) * rows*cols).  Backward: given lam = dL/dh_T
+ * (overwritten with dL/dh_0), accumulates dL/dz into lam_z and returns
+ * dL/dr, dL/deps; work buffers are allocated internally. */
+int calhipSciddicaTForwardDev(int rows, int cols, const double* d_z, const double* d_h0,
+                              double p_r, double p_eps, int steps, int ck,
+                              double* d_ckpt, double* d_h_out, double* timing_ms);
+int calhipSciddicaTAdjointDev(int rows, int cols, const double* d_z, const double* d_ckpt,
+                              double p_r, double p_eps, int steps, int ck,
+                              double* d_lam, double* d_lam_z, double* d_r, double* d_eps,
+                              double* timing_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

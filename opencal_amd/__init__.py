@@ -1,0 +1,119 @@
+"""opencal_amd: an MI355X engine for OpenCAL's 2D cellular-automata hot path.
+
+The product is the C-ABI library ``libopencal_hip.so`` (headers in
+``include/``): OpenCAL's 2D API with substates resident on the GPU and
+hand-written HIP device programs for SciddicaT, Life and SCIARA.  This package
+loads it with ctypes for tests, benchmarks and the torch.autograd wrapper of
+the SciddicaT adjoint.  It never falls back to a CPU path: when the library is
+missing, loading fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libopencal_hip.so")
+EXAMPLES_PATH = os.path.join(HERE, "libopencal_examples.so")
+
+_lib = None
+_ex = None
+
+
+def _import_torch_first():
+    # torch ships its own HIP runtime (same soname as ROCm's); loading torch
+    # before the engine makes both use that one runtime in this process.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
+def lib() -> C.CDLL:
+    """The engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        _import_torch_first()
+        _lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        _declare(_lib)
+    return _lib
+
+
+def examples() -> C.CDLL:
+    """The OpenCAL example models built against the engine."""
+    global _ex
+    if _ex is None:
+        lib()
+        if not os.path.exists(EXAMPLES_PATH):
+            raise RuntimeError(f"{EXAMPLES_PATH} is not built")
+        _ex = C.CDLL(EXAMPLES_PATH)
+        _declare_examples(_ex)
+    return _ex
+
+
+class CalhipStats(C.Structure):
+    _fields_ = [("steps", C.c_longlong), ("launches", C.c_longlong), ("uploads", C.c_longlong),
+                ("downloads", C.c_longlong), ("rebuilds", C.c_longlong), ("materialisations", C.c_longlong)]
+
+
+def _declare(L):
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.calhipGetLastError.restype = C.c_int
+    L.calhipGetErrorString.restype = C.c_char_p
+    L.calCADef2D.restype = vp
+    L.calCADef2D.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.calFinalize2D.argtypes = [vp]
+    L.calhipGetStats2D.argtypes = [vp, C.POINTER(CalhipStats)]
+    L.calhipEnableKernelTiming2D.argtypes = [vp, C.c_int]
+    L.calhipGetKernelTime2D.argtypes = [vp, dp, C.POINTER(C.c_longlong)]
+    L.calhipSynchronize2D.argtypes = [vp]
+    L.calhipSyncHost2D.argtypes = [vp]
+    L.calhipIsOnDevice2D.argtypes = [vp]
+    L.calhipSciddicaTBatchRun.argtypes = [C.c_int, ip, ip, dp, dp, C.POINTER(dp), C.POINTER(dp), C.c_int,
+                                          C.POINTER(dp), dp]
+    L.calhipSciddicaTForwardDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                            vp, vp, dp]
+    L.calhipSciddicaTAdjointDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                            vp, vp, dp, dp, dp]
+
+
+def _declare_examples(E):
+    vp, dp, ip, bp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_ubyte)
+    E.sciddicat_create.restype = vp
+    E.sciddicat_create.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_double, C.c_int, C.c_int]
+    for name in ("sciddicat_run", "sciddicat_init", "sciddicat_sync"):
+        getattr(E, name).argtypes = [vp]
+    E.sciddicat_steps.argtypes = [vp, C.c_int]
+    E.sciddicat_read.argtypes = [vp, dp, dp, ip]
+    E.sciddicat_read_flow.argtypes = [vp, C.c_int, dp]
+    E.sciddicat_model.restype = vp
+    E.sciddicat_model.argtypes = [vp]
+    E.sciddicat_destroy.argtypes = [vp]
+    E.life_create.restype = vp
+    E.life_create.argtypes = [C.c_int, C.c_int, bp, C.c_int]
+    E.life_run.argtypes = [vp]
+    E.life_steps.argtypes = [vp, C.c_int]
+    E.life_read.argtypes = [vp, bp]
+    E.life_model.restype = vp
+    E.life_model.argtypes = [vp]
+    E.life_destroy.argtypes = [vp]
+
+
+def last_error() -> tuple[int, str]:
+    L = lib()
+    return L.calhipGetLastError(), L.calhipGetErrorString().decode()
+
+
+def check(rc=0, what="opencal_amd"):
+    code, msg = last_error()
+    if rc or code:
+        raise RuntimeError(f"{what}: rc={rc} error={code}: {msg}")
+
+
+def stats(model_ptr) -> dict:
+    s = CalhipStats()
+    lib().calhipGetStats2D(model_ptr, C.byref(s))
+    return {k: getattr(s, k) for k, _ in s._fields_}

@@ -1,0 +1,163 @@
+// Private state of the MI355X OpenCAL engine.
+//
+// A model is a ModelX whose first member is the public CALModel2D, and every
+// substate is a SubX whose first member is the public CALSubstate2D{b,i,r};
+// the public pointers handed to user code are the addresses of those first
+// members.
+//
+// Residency.  A model is either in the HOST phase (the host mirrors
+// Q->current / Q->next / A.* are authoritative and every API call runs the
+// reference semantics on them) or in the DEVICE phase (the device buffers are
+// authoritative).  The global transition of a bound device program moves the
+// model to the DEVICE phase; whole-substate operations (init, update, active
+// set rebuild) stay on the device; per-cell host accessors bring the model
+// back to the HOST phase first.
+//
+// Device layers.  Each double-layer substate has two device buffers.  After a
+// program step or a calUpdate2D the next layer is logically equal to current
+// (OpenCAL copies next->current after every process) and is physically stale:
+// `nxt_stale`.  A partial write to a stale next layer first copies current
+// into it.  `nxt_dirty` marks a next layer written since the last update; the
+// update then flips the two buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+extern "C" {
+#include "cal2D.h"
+#include "cal2DRun.h"
+#include "calhip2D.h"
+}
+
+namespace calhip {
+
+enum class Phase { Host, Device };
+
+// Transient substate states (flows of a fused program, see calhip2D.h).
+enum class Transient {
+	None,          // ordinary substate
+	Zero,          // both layers logically 0 everywhere; buffers not written
+	Derived,       // logically the flows of the last fused step (not stored)
+	PendingZero,   // next layer reset to 0 by the steering, awaiting the update
+	ZeroAtActive,  // physical buffer valid except: logically 0 at the active cells
+	DerivedActive, // physical buffer holds the last flows (aliased layers)
+	PendingActive  // next layer reset at the active cells, awaiting the update
+};
+
+struct ModelX;
+struct SubX;
+
+// A device program replaces the model's elementary processes (and may own
+// transient substates).  Implementations live next to their kernels.
+struct Program {
+	virtual ~Program() {}
+	virtual const char* name() const = 0;
+	// calGlobalTransitionFunction2D on the device.
+	virtual int step(ModelX* m) = 0;
+	// Several steps when no host callback runs in between (default: loop).
+	virtual int run_steps(ModelX* m, int n) {
+		for (int k = 0; k < n; ++k)
+			if (int e = step(m)) return e;
+		return 0;
+	}
+	// calInitSubstate(Next)2Dr on a substate the program marked transient.
+	// Returns 1 when handled, 0 to fall back to an ordinary device fill (after
+	// materialising), <0 on error.
+	virtual int transient_init(ModelX*, SubX*, double, bool /*both*/) { return 0; }
+	// calUpdate2D hook for transient substates.
+	virtual void transient_update(ModelX*) {}
+	// Write a transient substate's logical content into its device buffers.
+	virtual int materialise(ModelX*, SubX*) { return -1; }
+	// A non-program operation is about to modify `s` on the device.
+	virtual int before_modify(ModelX*, SubX*) { return 0; }
+	// The model has just been uploaded (host mirrors are the logical state).
+	virtual int after_upload(ModelX*) { return 0; }
+};
+
+struct SubX {
+	union {
+		CALSubstate2Db b;
+		CALSubstate2Di i;
+		CALSubstate2Dr r;
+	} pub;
+	char type = 'r';   // 'b', 'i' or 'r'
+	int elem = 8;      // bytes per cell
+	bool single = false;
+	bool registered = false;  // listed in pQ*_array (double-layer)
+	void* d_cur = nullptr;
+	void* d_nxt = nullptr;
+	bool nxt_stale = false;
+	bool nxt_dirty = false;
+	Transient tstate = Transient::None;
+
+	void* h_cur() const { return (void*)pub.r.current; }
+	void* h_nxt() const { return (void*)pub.r.next; }
+};
+
+struct Timer {
+	bool enabled = false;
+	double total_ms = 0.0;
+	long long launches = 0;
+	std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct ModelX {
+	CALModel2D pub;
+	std::vector<SubX*> subs;   // every substate, creation order
+	Phase phase = Phase::Host;
+	hipStream_t stream = nullptr;
+	int device = 0;
+
+	// device active set: flags (rows*cols bytes), row-major list of linear
+	// indices, count[0] = list length (0 <=> OpenCAL's A.cells == NULL).
+	uint8_t* d_flags = nullptr;
+	int* d_cells = nullptr;
+	int* d_count = nullptr;
+	int* d_block_counts = nullptr;
+	int n_count_blocks = 0;
+	bool flags_dirty = false;  // flags changed since the last rebuild
+
+	Program* prog = nullptr;
+	std::vector<CALCallbackFunc2D> prog_eps;  // processes the program covers
+
+	calhipStats stats{};
+	Timer timer;
+
+	size_t cells() const { return (size_t)pub.rows * (size_t)pub.columns; }
+	bool active() const { return pub.OPTIMIZATION == CAL_OPT_ACTIVE_CELLS; }
+};
+
+inline ModelX* X(CALModel2D* m) { return reinterpret_cast<ModelX*>(m); }
+inline SubX* S(void* q) { return reinterpret_cast<SubX*>(q); }
+
+// ---- error handling (never exits the process) ----
+void set_error(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+#define CALHIP_CHECK(call) ::calhip::check_hip((call), #call)
+
+// ---- residency ----
+int ensure_host(ModelX* m);    // DEVICE -> HOST (no-op in HOST)
+int ensure_device(ModelX* m);  // HOST -> DEVICE (no-op in DEVICE)
+int ensure_device_buffers(ModelX* m, SubX* s);
+int materialise_next(ModelX* m, SubX* s);  // stale next := current
+int device_fill(ModelX* m, SubX* s, double v, bool both);
+int device_update(ModelX* m);
+int device_rebuild_active(ModelX* m);
+void flip(SubX* s);
+
+// ---- kernel launchers (defined in the .hip files) ----
+namespace k {
+int fill(ModelX* m, void* buf, int elem, double v, bool active_only);
+int copy(ModelX* m, void* dst, const void* src, size_t bytes);
+int rebuild_active(ModelX* m);
+int active_count_host(ModelX* m, int* out);  // synchronous read of count[0]
+}  // namespace k
+
+// kernel timing helpers
+void timer_begin(ModelX* m, hipEvent_t* ev0);
+void timer_end(ModelX* m, hipEvent_t ev0);
+void timer_collect(ModelX* m);
+
+}  // namespace calhip

@@ -1,0 +1,154 @@
+"""Host-side drivers of the OpenCAL example models built against the engine
+(opencal_amd/examples/*.c), used by tests and benchmarks.  Every call goes
+through the C ABI of libopencal_hip.so."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import check, examples, lib, stats
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_bp = C.POINTER(C.c_ubyte)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+class SciddicaT:
+    """One SciddicaT model (full grid or active cells) on the GPU.
+
+    z, h: altitude and thickness before the model's init process (the init
+    lowers z by h where h > 0, as the reference example's init does)."""
+
+    def __init__(self, z, h, p_r, p_eps, steps, active=False):
+        self.z = np.ascontiguousarray(z, np.float64)
+        self.h = np.ascontiguousarray(h, np.float64)
+        self.rows, self.cols = self.z.shape
+        self.E = examples()
+        lib().calhipClearError()
+        self.handle = self.E.sciddicat_create(self.rows, self.cols, _d(self.z), _d(self.h), float(p_r),
+                                              float(p_eps), int(steps), int(bool(active)))
+        if not self.handle:
+            check(-1, "sciddicat_create")
+
+    @property
+    def model(self):
+        return self.E.sciddicat_model(self.handle)
+
+    def run(self):
+        check(self.E.sciddicat_run(self.handle), "sciddicat_run")
+        return self
+
+    def init(self):
+        check(self.E.sciddicat_init(self.handle), "sciddicat_init")
+
+    def steps(self, n):
+        check(self.E.sciddicat_steps(self.handle, int(n)), "sciddicat_steps")
+
+    def sync(self):
+        check(self.E.sciddicat_sync(self.handle), "sciddicat_sync")
+
+    def read(self):
+        oz = np.empty((self.rows, self.cols), np.float64)
+        oh = np.empty_like(oz)
+        cnt = C.c_int(0)
+        check(self.E.sciddicat_read(self.handle, _d(oz), _d(oh), C.byref(cnt)), "sciddicat_read")
+        return oz, oh, cnt.value
+
+    def read_flow(self, k):
+        out = np.empty((self.rows, self.cols), np.float64)
+        check(self.E.sciddicat_read_flow(self.handle, int(k), _d(out)), "sciddicat_read_flow")
+        return out
+
+    def stats(self):
+        return stats(self.model)
+
+    def timing(self, enable):
+        lib().calhipEnableKernelTiming2D(self.model, int(enable))
+
+    def kernel_time(self):
+        ms = C.c_double()
+        n = C.c_longlong()
+        lib().calhipGetKernelTime2D(self.model, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def close(self):
+        if self.handle:
+            self.E.sciddicat_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sciddicat_run(z, h, p_r, p_eps, steps, active=False):
+    """Run `steps` steps; returns (z, h, active_count) after the run."""
+    m = SciddicaT(z, h, p_r, p_eps, steps, active)
+    try:
+        m.run()
+        return m.read()
+    finally:
+        m.close()
+
+
+class Life:
+    def __init__(self, alive0, steps):
+        self.a0 = np.ascontiguousarray(alive0, np.uint8)
+        self.rows, self.cols = self.a0.shape
+        self.E = examples()
+        lib().calhipClearError()
+        self.handle = self.E.life_create(self.rows, self.cols, self.a0.ctypes.data_as(_bp), int(steps))
+        if not self.handle:
+            check(-1, "life_create")
+
+    @property
+    def model(self):
+        return self.E.life_model(self.handle)
+
+    def run(self):
+        check(self.E.life_run(self.handle), "life_run")
+        return self
+
+    def steps(self, n):
+        check(self.E.life_steps(self.handle, int(n)), "life_steps")
+
+    def read(self):
+        out = np.empty((self.rows, self.cols), np.uint8)
+        check(self.E.life_read(self.handle, out.ctypes.data_as(_bp)), "life_read")
+        return out
+
+    def timing(self, enable):
+        lib().calhipEnableKernelTiming2D(self.model, int(enable))
+
+    def kernel_time(self):
+        ms = C.c_double()
+        n = C.c_longlong()
+        lib().calhipGetKernelTime2D(self.model, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def close(self):
+        if self.handle:
+            self.E.life_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def life_run(alive0, steps):
+    m = Life(alive0, steps)
+    try:
+        m.run()
+        return m.read()
+    finally:
+        m.close()

@@ -1,0 +1,132 @@
+"""ctypes access to the oracle (oracle/_build/liboracle.so) and, when it was
+built in this tree, to the reference build (oracle/_ref).  Test-side only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_bp = C.POINTER(C.c_ubyte)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def bptr(a: np.ndarray):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(_bp)
+
+
+_cache: dict = {}
+
+
+def _load(path):
+    if path not in _cache:
+        _cache[path] = C.CDLL(path)
+    return _cache[path]
+
+
+def oracle():
+    if not os.path.exists(ORACLE_SO):
+        import subprocess
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _load(ORACLE_SO)
+
+
+def ref(name: str):
+    """The reference build `libref_<name>.so`, or None when it is absent."""
+    p = os.path.join(REF_DIR, f"libref_{name}.so")
+    return _load(p) if os.path.exists(p) else None
+
+
+# ---------------------------------------------------------------- SciddicaT
+
+def oracle_sciddicat(z, h, p_r, p_eps, steps, active=False, counts=False):
+    z = np.ascontiguousarray(z, np.float64)
+    h = np.ascontiguousarray(h, np.float64)
+    rows, cols = z.shape
+    oz, oh = np.empty_like(z), np.empty_like(h)
+    ac = np.zeros(max(steps, 1), np.int32)
+    rc = oracle().orc_sciddicat_run(rows, cols, dptr(z), dptr(h), C.c_double(p_r),
+                                    C.c_double(p_eps), steps, int(active), dptr(oz), dptr(oh),
+                                    iptr(ac) if counts else None)
+    assert rc == 0
+    return (oz, oh, ac) if counts else (oz, oh)
+
+
+def ref_sciddicat(z, h, p_r, p_eps, steps, active=False, counts=False):
+    lib = ref("sciddicat_active" if active else "sciddicat")
+    if lib is None:
+        return None
+    z = np.ascontiguousarray(z, np.float64)
+    h = np.ascontiguousarray(h, np.float64)
+    rows, cols = z.shape
+    oz, oh = np.empty_like(z), np.empty_like(h)
+    ac = np.zeros(max(steps, 1), np.int32)
+    if active:
+        rc = lib.ref_sciddicat_active_run(rows, cols, dptr(z), dptr(h), C.c_double(p_r),
+                                          C.c_double(p_eps), steps, dptr(oz), dptr(oh), iptr(ac))
+    else:
+        rc = lib.ref_sciddicat_run(rows, cols, dptr(z), dptr(h), C.c_double(p_r),
+                                   C.c_double(p_eps), steps, dptr(oz), dptr(oh))
+    assert rc == 0
+    return (oz, oh, ac) if counts else (oz, oh)
+
+
+def oracle_step(z, h, p_r, p_eps):
+    z = np.ascontiguousarray(z, np.float64)
+    h = np.ascontiguousarray(h, np.float64)
+    out = np.empty_like(h)
+    oracle().orc_sciddicat_step(h.shape[0], h.shape[1], dptr(z), dptr(h), C.c_double(p_r),
+                                C.c_double(p_eps), dptr(out))
+    return out
+
+
+def oracle_adjoint(z, h0, p_r, p_eps, steps, lam_T):
+    z = np.ascontiguousarray(z, np.float64)
+    h0 = np.ascontiguousarray(h0, np.float64)
+    lam_T = np.ascontiguousarray(lam_T, np.float64)
+    lh, lz = np.empty_like(h0), np.empty_like(h0)
+    dr, de = C.c_double(), C.c_double()
+    oracle().orc_sciddicat_adjoint(h0.shape[0], h0.shape[1], dptr(z), dptr(h0), C.c_double(p_r),
+                                   C.c_double(p_eps), steps, dptr(lam_T), dptr(lh), dptr(lz),
+                                   C.byref(dr), C.byref(de))
+    return lh, lz, dr.value, de.value
+
+
+# --------------------------------------------------------------------- Life
+
+def oracle_life(a0, steps):
+    a0 = np.ascontiguousarray(a0, np.uint8)
+    out = np.empty_like(a0)
+    assert oracle().orc_life_run(a0.shape[0], a0.shape[1], bptr(a0), steps, bptr(out)) == 0
+    return out
+
+
+def ref_life(a0, steps):
+    lib = ref("life")
+    if lib is None:
+        return None
+    a0 = np.ascontiguousarray(a0, np.uint8)
+    out = np.empty_like(a0)
+    assert lib.ref_life_run(a0.shape[0], a0.shape[1], bptr(a0), steps, bptr(out)) == 0
+    return out
+
+
+def set_threads(n: int) -> int:
+    return oracle().orc_set_threads(n)

@@ -1,0 +1,108 @@
+"""SciddicaT on the GPU through the C ABI, checked against the oracle.
+
+Bar: bit-exact (FP64 with the reference's operation order, no contraction).
+Cases: seeded synthetic DEMs of several (ragged) sizes, full grid and active
+cells, the reference's shipped 610x496x4000 result, the transient-flow
+bookkeeping, and mass conservation at the 4096^2 benchmark size."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+import native
+from opencal_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def gpu_run(z, h, p_r, p_eps, steps, active):
+    from opencal_amd.models import sciddicat_run
+    return sciddicat_run(z, h, p_r, p_eps, steps, active)
+
+
+@pytest.mark.parametrize("active", [False, True])
+@pytest.mark.parametrize("rows,cols,steps,seed", [
+    (64, 96, 300, 1), (97, 131, 200, 2), (128, 61, 250, 3), (200, 256, 150, 4), (3, 5, 20, 5),
+])
+def test_bit_exact_vs_oracle(rows, cols, steps, seed, active):
+    print(f"case rows={rows} cols={cols} steps={steps} seed={seed} active={active}", flush=True)
+    z, h, _ = synth.sciddicat_inputs(rows, cols, seed, patch=min(16, rows // 2 or 1))
+    gz, gh, gcount = gpu_run(z, h, 0.5, 0.001, steps, active)
+    oz, oh, ocount = native.oracle_sciddicat(z, h, 0.5, 0.001, steps, active, counts=True)
+    assert np.array_equal(gz, oz)
+    assert np.array_equal(gh, oh), f"max |dh| = {np.abs(gh - oh).max()}"
+    if active:
+        assert gcount == ocount[-1]
+
+
+@pytest.mark.parametrize("p_r,p_eps", [(0.3, 1e-4), (0.7, 1e-2), (0.61, 0.0031)])
+def test_parameters(p_r, p_eps):
+    z, h, _ = synth.sciddicat_inputs(80, 72, 9, patch=12)
+    for active in (False, True):
+        gz, gh, _ = gpu_run(z, h, p_r, p_eps, 120, active)
+        oz, oh = native.oracle_sciddicat(z, h, p_r, p_eps, 120, active)
+        assert np.array_equal(gh, oh)
+
+
+def test_reference_width_final_full_grid():
+    """OpenCAL-OMP/examples/calomp_sciddicaT/data/width_final.txt, 4000 steps,
+    written with "%.9f" exactly as the reference saves it."""
+    z = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_dem.txt.gz"))
+    h = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_source.txt.gz"))
+    _, gh, _ = gpu_run(z, h, 0.5, 0.001, 4000, False)
+    text = gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
+    mine = "".join("".join("%.9f  " % v for v in row) + "\n" for row in gh)
+    assert mine == text
+
+
+def test_reference_width_final_active():
+    """cal_sciddicaT-activecells-glut/data/width_final.txt ("%f"), 4000 steps."""
+    z = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_dem.txt.gz"))
+    h = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_source.txt.gz"))
+    _, gh, _ = gpu_run(z, h, 0.5, 0.001, 4000, True)
+    ref = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_active_610x496_width_final.txt.gz"))
+    assert np.array_equal(np.round(gh, 6), ref) or np.abs(gh - ref).max() <= 5e-7
+
+
+def test_flows_observable_after_step():
+    """Flows are not stored by the fused sweep; reading them writes them out
+    and must give the reference's values (the steering has reset them: 0)."""
+    from opencal_amd.models import SciddicaT
+    z, h, _ = synth.sciddicat_inputs(64, 64, 21, patch=12)
+    m = SciddicaT(z, h, 0.5, 0.001, 10, False)
+    m.init()
+    m.steps(10)
+    f0 = m.read_flow(0)
+    assert np.all(f0 == 0.0)
+    _, gh, _ = m.read()
+    _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 10, False)
+    assert np.array_equal(gh, oh)
+    # continue after the host round trip: state re-uploaded, still exact
+    m.steps(15)
+    _, gh2, _ = m.read()
+    _, oh2 = native.oracle_sciddicat(z, h, 0.5, 0.001, 25, False)
+    assert np.array_equal(gh2, oh2)
+    st = m.stats()
+    assert st["uploads"] == 2 and st["downloads"] >= 2
+    m.close()
+
+
+def test_mass_conservation_4096():
+    """At the benchmark size: total thickness is conserved on the torus (to
+    rounding), and the first 3 steps match the oracle bit for bit."""
+    from opencal_amd.models import SciddicaT
+    z, h, _ = synth.sciddicat_inputs(4096, 4096, 2024)
+    m = SciddicaT(z, h, 0.5, 0.001, 200, False)
+    m.init()
+    m.steps(3)
+    _, gh3, _ = m.read()
+    _, oh3 = native.oracle_sciddicat(z, h, 0.5, 0.001, 3, False)
+    assert np.array_equal(gh3, oh3)
+    m.steps(197)
+    _, gh, _ = m.read()
+    assert abs(gh.sum() - h.sum()) <= 1e-9 * h.sum()
+    assert gh.min() >= 0.0
+    m.close()
