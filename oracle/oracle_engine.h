@@ -5,6 +5,7 @@
  * Reference: OpenCAL-OMP/source/cal2D.c, cal2DBuffer.c, include/calCommon.h. */
 #ifndef OPENCAL_ORACLE_ENGINE_H
 #define OPENCAL_ORACLE_ENGINE_H
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -93,7 +94,15 @@ static inline void orc_update_sub(const orc_grid* g, orc_sub_r* q)
 				q->cur[c] = q->nxt[c];
 		}
 	} else {
-		memcpy(q->cur, q->nxt, sizeof(double) * (size_t)g->rows * g->cols);
+		/* chunked parallel memcpy, one chunk per thread (cal2DBuffer.c:103-133) */
+		size_t n = (size_t)g->rows * g->cols;
+#pragma omp parallel
+		{
+			int t = omp_get_thread_num(), nt = omp_get_num_threads();
+			size_t chunk = n / (size_t)nt, start = chunk * (size_t)t;
+			if (t == nt - 1) chunk = n - start;
+			memcpy(q->cur + start, q->nxt + start, sizeof(double) * chunk);
+		}
 	}
 }
 
