@@ -1,18 +1,25 @@
 """Benchmark of the OpenCAL hot path on one MI355X.
 
-Headline workload: SciddicaT debris flow, FP64, 4096 x 4096 toroidal grid,
-full grid (CAL_NO_OPT), driven through OpenCAL's C API (calRunCAStep2D per
-step: global transition on the GPU, host steering callback, implicit update).
-Synthetic input: 500 m slope + 20 Gaussian bumps, 40 x 40 source patch.
+Headline (`value`): SciddicaT debris flow, FP64, 4096 x 4096 toroidal grid,
+full grid (CAL_NO_OPT), driven through OpenCAL's C API exactly as the
+reference example drives it (calRunCAStep2D per step: global transition on the
+GPU, the model's host steering callback, implicit update), with the model
+already resident in HBM.  Unit: giga cell updates per second.
 
-value = cell updates per second (GCUPS) over the timed steps with the model
-already resident in HBM.  The same JSON line carries the roofline of the
-fused sweep kernel (HIP events on the model's stream), the CPU baseline (the
-reference OpenCAL-OMP build when present, else the oracle restatement) timed
-on this host, and the secondary workloads (active cells, Life, batched
-ensemble, adjoint) when --all is given.
+The same JSON line carries
+  roofline      the fused sweep kernel's achieved algorithmic bandwidth
+                (24 B per cell update: z read, h read, h written) from HIP
+                events on the model's stream, against the 8 TB/s HBM peak;
+  cpu_baseline  the reference OpenCAL-OMP build (oracle/_ref, when present;
+                else the oracle restatement) timed on this host's cores on a
+                bounded sample of the same workload;
+  workloads     the other measured configurations: SciddicaT active cells
+                1024^2 x 4000, Life 512^2 x 1000, the batched ensemble of 256
+                uneven instances x 2000 steps against the same instances run
+                one after another, and the adjoint gradient / forward time
+                ratio at 1024^2 x 2000 with checkpoints every 45 steps.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--all]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
 from __future__ import annotations
 
@@ -36,13 +43,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# --------------------------------------------------------------- headline
+
 def bench_sciddicat_plain(n, steps, warmup, seed=2024):
     from opencal_amd import synth
     from opencal_amd.models import SciddicaT
     z, h, _ = synth.sciddicat_inputs(n, n, seed)
     m = SciddicaT(z, h, 0.5, 0.001, steps + warmup, active=False)
     m.init()
-    m.steps(warmup)          # first step uploads the model
+    m.steps(warmup)          # the first step uploads the model
     m.sync()
     m.timing(True)
     t0 = time.perf_counter()
@@ -56,17 +65,16 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024):
     mass_err = abs(hh.sum() - h.sum()) / h.sum()
     m.close()
     cells = n * n
+    kavg = ktime_ms / max(launches, 1)
     return {
-        "wall_s": wall, "gcups": cells * steps / wall / 1e9,
-        "kernel_ms_avg": ktime_ms / max(launches, 1), "kernel_launches": launches,
-        "achieved_gbs": BYTES_PER_CELL_SCIDDICAT * cells / (ktime_ms / max(launches, 1) * 1e-3) / 1e9,
+        "wall_s": wall, "gcups": cells * steps / wall / 1e9, "kernel_ms_avg": kavg, "kernel_launches": launches,
+        "achieved_gbs": BYTES_PER_CELL_SCIDDICAT * cells / (kavg * 1e-3) / 1e9,
         "launches_per_step": st["launches"] / max(st["steps"], 1), "mass_rel_err": mass_err,
+        "uploads": st["uploads"],
     }
 
 
 def cpu_baseline_sciddicat(n, steps, seed=2024):
-    """Reference OpenCAL-OMP (oracle/_ref build) or the oracle restatement,
-    full grid, timed on this host's cores."""
     import native
     from opencal_amd import synth
     z, h, _ = synth.sciddicat_inputs(n, n, seed)
@@ -77,8 +85,141 @@ def cpu_baseline_sciddicat(n, steps, seed=2024):
     t0 = time.perf_counter()
     fn(z, h, 0.5, 0.001, steps)
     wall = time.perf_counter() - t0
-    return {"value": n * n * steps / wall / 1e9, "unit": "GCUPS", "cores": threads, "kind": kind,
-            "sample": f"SciddicaT {n}x{n} full grid, {steps} steps (incl. init), OMP_NUM_THREADS={threads}"}
+    return {"value": round(n * n * steps / wall / 1e9, 5), "unit": "GCUPS", "cores": threads, "kind": kind,
+            "sample": f"SciddicaT {n}x{n} full grid, {steps} steps incl. init, OMP_NUM_THREADS={threads}, "
+                      f"{wall:.1f} s"}
+
+
+# ----------------------------------------------------------- other configs
+
+def bench_active(n=1024, steps=4000, seed=7, cpu_steps=400):
+    """SciddicaT with CAL_OPT_ACTIVE_CELLS (config 2), stepped through the C API."""
+    import native
+    from opencal_amd import synth
+    from opencal_amd.models import SciddicaT
+    z, h, _ = synth.sciddicat_inputs(n, n, seed)
+    m = SciddicaT(z, h, 0.5, 0.001, steps, active=True)
+    m.init()
+    m.steps(1)
+    m.sync()
+    t0 = time.perf_counter()
+    m.steps(steps - 1)
+    m.sync()
+    wall = time.perf_counter() - t0
+    _, gh, count = m.read()
+    m.close()
+    _, oh, counts = native.oracle_sciddicat(z, h, 0.5, 0.001, steps, True, counts=True)
+    active_updates = float(np.sum(counts[1:steps].astype(np.float64)))
+    out = {"workload": f"sciddicaT_active_{n}x{n}_x{steps}", "ms_per_step": wall / (steps - 1) * 1e3,
+           "active_gcups": active_updates / wall / 1e9, "full_grid_equiv_gcups": n * n * (steps - 1) / wall / 1e9,
+           "final_active_cells": count, "bit_exact_vs_oracle": bool(np.array_equal(gh, oh) and count == counts[-1])}
+    ref = native.ref("sciddicat_active")
+    if ref is not None:
+        t0 = time.perf_counter()
+        native.ref_sciddicat(z, h, 0.5, 0.001, cpu_steps, True)
+        cw = time.perf_counter() - t0
+        out["cpu_reference_ms_per_step"] = cw / cpu_steps * 1e3
+        out["cpu_sample"] = f"reference active-cells build, first {cpu_steps} steps incl. init"
+    return out
+
+
+def bench_life(n=512, steps=1000, seed=1):
+    import native
+    from opencal_amd import synth
+    from opencal_amd.models import Life
+    a0 = synth.life_inputs(n, n, seed)
+    res = {}
+    for mode in ("run", "steps"):
+        m = Life(a0, steps)
+        t0 = time.perf_counter()
+        if mode == "run":
+            m.run()             # calRun2D: no callbacks between steps -> LDS-resident multi-step
+        else:
+            m.steps(steps)      # one calRunCAStep2D (one launch) per step
+        got = m.read()
+        wall = time.perf_counter() - t0
+        m.close()
+        res[mode] = (wall, got)
+    want = native.oracle_life(a0, steps)
+    alive = int(want.sum())
+    return {"workload": f"life_{n}x{n}_x{steps}_uint8",
+            "calRun2D_us_per_step": res["run"][0] / steps * 1e6,
+            "calRun2D_gcups": n * n * steps / res["run"][0] / 1e9,
+            "per_step_launch_us_per_step": res["steps"][0] / steps * 1e6,
+            "alive": alive, "bit_exact_vs_oracle": bool(np.array_equal(res["run"][1], want) and
+                                                        np.array_equal(res["steps"][1], want)),
+            "note": "wall time incl. upload/download of the 256 KiB board"}
+
+
+def bench_batch(n_inst=256, steps=2000, seed=4):
+    from opencal_amd import synth
+    from opencal_amd.models import SciddicaT, sciddicat_batch
+    specs = synth.batch_instances(n_inst, seed)
+    insts = []
+    for rows, cols, pr, pe, s in specs:
+        z, h, _ = synth.sciddicat_inputs(rows, cols, s)
+        insts.append((z, h, pr, pe))
+    cells = sum(r * c for r, c, *_ in specs)
+    log(f"batch: {n_inst} instances, {cells / 1e6:.1f} M cells")
+    outs, ms = sciddicat_batch(insts, steps)
+    # the same instances one after another through the single-model C API
+    seq_s = 0.0
+    exact = True
+    for (z, h, pr, pe), got in zip(insts, outs):
+        m = SciddicaT(z, h, pr, pe, steps, False)
+        m.init()
+        m.steps(1)
+        m.sync()
+        t0 = time.perf_counter()
+        m.steps(steps - 1)
+        m.sync()
+        seq_s += (time.perf_counter() - t0) * steps / (steps - 1)
+        _, hh, _ = m.read()
+        m.close()
+        exact = exact and bool(np.array_equal(hh, got))
+    bat_gcups = cells * steps / (ms * 1e-3) / 1e9
+    seq_gcups = cells * steps / seq_s / 1e9
+    return {"workload": f"sciddicaT_batch_{n_inst}_instances_128..1024_x{steps}", "cells": cells,
+            "batched_gcups": bat_gcups, "sequential_gcups": seq_gcups, "speedup": bat_gcups / seq_gcups,
+            "batched_s": ms * 1e-3, "sequential_s": seq_s, "each_instance_equals_its_single_run": exact}
+
+
+def bench_adjoint(n=1024, steps=2000, ck=45, seed=5):
+    import torch
+    from opencal_amd import synth
+    from opencal_amd.adjoint import forward_only, sciddicat_run
+    z_np, h_np, (r0, c0, p) = synth.sciddicat_inputs(n, n, seed)
+    dev = "cuda"
+    z_dem = torch.tensor(z_np, dtype=torch.float64, device=dev)
+    patch0 = torch.tensor(h_np[r0:r0 + p, c0:c0 + p].copy(), dtype=torch.float64, device=dev)
+    target = forward_only(z_dem, patch0, (r0, c0), 0.6, 0.003, steps)      # hidden parameters
+    forward_only(z_dem, patch0, (r0, c0), 0.5, 0.001, 10)                  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    forward_only(z_dem, patch0, (r0, c0), 0.5, 0.001, steps)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t0
+    pr = torch.tensor(0.5, dtype=torch.float64, requires_grad=True)
+    pe = torch.tensor(0.001, dtype=torch.float64, requires_grad=True)
+    patch = patch0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hT = sciddicat_run(pr, pe, patch, z_dem, (r0, c0), steps, ck)
+    loss = ((hT - target) ** 2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    grad_s = time.perf_counter() - t0
+
+    def L(prv):
+        with torch.no_grad():
+            return ((forward_only(z_dem, patch0, (r0, c0), prv, 0.001, steps) - target) ** 2).sum().item()
+    h = 1e-6
+    fd = (L(0.5 + h) - L(0.5 - h)) / (2 * h)
+    g = pr.grad.item()
+    return {"workload": f"sciddicaT_adjoint_{n}x{n}_x{steps}_ck{ck}", "forward_s": fwd_s,
+            "gradient_s": grad_s, "gradient_over_forward": grad_s / fwd_s, "loss": loss.item(),
+            "dL_dpr": g, "dL_dpeps": pe.grad.item(), "patch_grad_norm": patch.grad.norm().item(),
+            "fd_dL_dpr": fd, "fd_rel_err": abs(g - fd) / max(abs(fd), 1e-30)}
 
 
 def main():
@@ -87,13 +228,13 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--all", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on one MI355X, "
-              "multi-GPU sharding is a design note only)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on one MI355X; "
+              "how the path would shard is a design note)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -119,12 +260,24 @@ def main():
                    "driver": "calRunCAStep2D per step (C API)"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "step_fused", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
+                     "kernel": "step_fused2", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
                      "bytes_per_cell": BYTES_PER_CELL_SCIDDICAT},
     }
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline_sciddicat(args.size, args.cpu_steps)
+    if not args.quick:
+        wl = {}
+        for name, fn in (("active", bench_active), ("life", bench_life), ("batch", bench_batch),
+                         ("adjoint", bench_adjoint)):
+            log(f"{name} ...")
+            t0 = time.perf_counter()
+            try:
+                wl[name] = fn()
+            except Exception as e:   # report, do not hide: the line still carries the error
+                wl[name] = {"error": repr(e)}
+            log(f"{name}: {json.dumps(wl[name])} ({time.perf_counter() - t0:.1f} s)")
+        out["workloads"] = wl
     print(json.dumps(out), flush=True)
 
 

@@ -81,14 +81,17 @@ int calhipBindLife2D(struct CALModel2D* ca2D, CALCallbackFunc2D transition,
 
 /* --------------------------------------------------- batched SciddicaT */
 
-/* Run `steps` full-grid SciddicaT steps on `n` independent instances of
- * uneven size and parameters in one persistent launch.
+/* Run `steps` full-grid SciddicaT steps (von Neumann, toroidal, flows reset
+ * every step) on `n` independent instances of uneven size and parameters;
+ * each step of the whole ensemble is one launch over a flat tile list.
  *   rows[k], cols[k]   grid of instance k (each >= 3)
  *   p_r[k], p_eps[k]   parameters of instance k
- *   z_host[k], h_host[k]  host arrays (rows[k]*cols[k]) with the state AFTER
- *                      the reference init (z already lowered by h)
+ *   z_host[k], h_host[k]  host arrays (rows[k]*cols[k]) with altitude and
+ *                      thickness before the model's init process (the init
+ *                      z -= h where h > 0 of sciddicaT.c:104-130 is applied)
  *   h_out[k]           host arrays receiving the final thickness
- * timing_ms (optional): GPU time of the stepping launch(es) only. */
+ * timing_ms (optional): GPU time of the stepping launches only (inputs
+ * resident).  Each instance's result equals its own single-model run. */
 int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols,
                             const double* p_r, const double* p_eps,
                             const double* const* z_host, const double* const* h_host,
@@ -111,6 +114,15 @@ int calhipSciddicaTAdjointDev(int rows, int cols, const double* d_z, const doubl
                               double p_r, double p_eps, int steps, int ck,
                               double* d_lam, double* d_lam_z, double* d_r, double* d_eps,
                               double* timing_ms);
+
+/* As calhipSciddicaTAdjointDev, with dL/dz accumulated only for the cells of
+ * the box [box_r0, box_r0 + box_rows) x [box_c0, box_c0 + box_cols) (the
+ * source patch, whose altitude the init lowered by the initial thickness). */
+int calhipSciddicaTAdjointBoxDev(int rows, int cols, const double* d_z, const double* d_ckpt,
+                                 double p_r, double p_eps, int steps, int ck,
+                                 double* d_lam, double* d_lam_z, double* d_r, double* d_eps,
+                                 int box_r0, int box_c0, int box_rows, int box_cols,
+                                 double* timing_ms);
 
 #ifdef __cplusplus
 }

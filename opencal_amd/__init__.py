@@ -78,6 +78,8 @@ def _declare(L):
                                             vp, vp, dp]
     L.calhipSciddicaTAdjointDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                             vp, vp, dp, dp, dp]
+    L.calhipSciddicaTAdjointBoxDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                               vp, vp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
 
 
 def _declare_examples(E):

@@ -1,18 +1,338 @@
-// SciddicaT adjoint.
+// SciddicaT adjoint (reverse mode) with checkpointing and recomputation.
+//
+// Forward map of one full-grid step (flows reset every step, sciddicaT.c:36-140):
+//   h'(c) = h(c) + sum_n ( f_opp(n)(nb_n(c)) - f_n(c) ),
+//   f_n(x) = r (avg(x) - u_n(x)) for n in S(x) (non-eliminated, n >= 1), else 0,
+//   avg(x) = (m(x) + sum_{k in S(x) incl. 0 if kept} u_k(x)) / cnt(x),
+//   m = h - eps, u_0 = z + eps, u_n = z(nb_n) + h(nb_n); f = 0 when h <= eps.
+// With the elimination sets S frozen at their forward values (the map is
+// piecewise linear; the sets and the h <= eps test switch on measure-zero
+// sets), the adjoint of one step, for lam = dL/dh':
+//   g_n(x)  = lam(nb_n(x)) - lam(x)                       (dL/df_n(x))
+//   B(x)    = r sum_{n in S} g_n(x) / cnt(x)              (dL/d avg * d avg/d m)
+//   du_n(x) = -r g_n(x) + B(x)                            (dL/du_n(x), n in S)
+//   dL/dh(c) = lam(c) + [wet c] B(c) + sum_n [wet x, opp(n) in S(x)] du_opp(n)(x),  x = nb_n(c)
+//   dL/dz(c) += [wet c, 0 kept] B(c) + the same neighbour terms
+//   dL/dr   += sum_x sum_{n in S(x)} g_n(x) (avg(x) - u_n(x))
+//   dL/deps += sum_x B(x) (-1 + [0 kept in x])
+// The kernel gathers these per cell (one thread per cell; neighbour values come
+// through the caches).  dL/dr and dL/deps are summed per block across all
+// steps (each block only ever adds into its own slot, so the sums are
+// deterministic) and reduced on the host at the end.
+//
+// Checkpointing: the forward writes h every `ck` steps; the backward walks the
+// segments from the last to the first, recomputes the segment's states from
+// its checkpoint into a stash with the forward sweep kernel, then runs the
+// adjoint steps of the segment in reverse.  With ck ~ sqrt(steps) memory is
+// O(sqrt(steps)) grids and the forward is recomputed once.
+#include <string.h>
+
+#include <vector>
+
 #include "engine.hpp"
+#include "device_common.hpp"
+#include "sciddicat_device.hpp"
+
+namespace calhip {
+
+namespace {
+
+using namespace sct;
+
+constexpr int ADJ_BLOCK = 256;
+constexpr int ADJ_RPW = 8;
+
+struct Box {
+	int r0, c0, nr, nc;   // dL/dz is accumulated for cells inside this box
+};
+
+__device__ __forceinline__ void cell_state(const double* __restrict__ z, const double* __restrict__ h, int rows,
+                                           int cols, int i, int j, double eps, FlowState& s, size_t nb[5])
+{
+	const size_t c = (size_t)i * cols + j;
+	nb[0] = c;
+	nb[1] = (size_t)dev::wrap1(i - 1, rows) * cols + j;
+	nb[2] = (size_t)i * cols + dev::wrap1(j - 1, cols);
+	nb[3] = (size_t)i * cols + dev::wrap1(j + 1, cols);
+	nb[4] = (size_t)dev::wrap1(i + 1, rows) * cols + j;
+	const double h0 = h[c];
+	if (h0 <= eps) {
+		s.wet = false;
+		return;
+	}
+	flow_state(h0, z[c], z[nb[1]] + h[nb[1]], z[nb[2]] + h[nb[2]], z[nb[3]] + h[nb[3]], z[nb[4]] + h[nb[4]], eps,
+	           s);
+}
+
+__global__ __launch_bounds__(ADJ_BLOCK) void adjoint_step(const double* __restrict__ z,
+                                                          const double* __restrict__ h,
+                                                          const double* __restrict__ lam1,
+                                                          double* __restrict__ lam0, double* __restrict__ lam_z,
+                                                          double* __restrict__ partial, int rows, int cols,
+                                                          double r, double eps, Box box)
+{
+	__shared__ double red_r[ADJ_BLOCK / 64], red_e[ADJ_BLOCK / 64];
+	const size_t n = (size_t)rows * cols;
+	const size_t c = (size_t)blockIdx.x * ADJ_BLOCK + threadIdx.x;
+	double dr = 0.0, de = 0.0;
+	if (c < n) {
+		const int i = (int)(c / cols), j = (int)(c % cols);
+		FlowState s;
+		size_t nb[5];
+		cell_state(z, h, rows, cols, i, j, eps, s, nb);
+		const double lc = lam1[c];
+		double acc = lc, lz = 0.0;
+		if (s.wet) {
+			double sg = 0.0;
+#pragma unroll
+			for (int k = 1; k < 5; ++k)
+				if (!((s.elim >> k) & 1u)) {
+					const double g = lam1[nb[k]] - lc;
+					sg += g;
+					dr += g * (s.avg - s.u[k]);
+				}
+			// cnt == 0: rounding eliminated every level (h - eps below the
+			// resolution of the level sum); no flow depends on the average
+			const double B = s.cnt ? r * sg / s.cnt : 0.0;
+			acc += B;
+			const bool kept0 = !(s.elim & 1u);
+			if (kept0) lz += B;
+			de += kept0 ? 0.0 : -B;
+		}
+		// neighbour x = nb[k] sending towards c: c is x's neighbour opp(k)
+#pragma unroll
+		for (int k = 1; k < 5; ++k) {
+			const int opp = 5 - k;
+			const int xi = k == 1 ? dev::wrap1(i - 1, rows) : (k == 4 ? dev::wrap1(i + 1, rows) : i);
+			const int xj = k == 2 ? dev::wrap1(j - 1, cols) : (k == 3 ? dev::wrap1(j + 1, cols) : j);
+			FlowState sx;
+			size_t nbx[5];
+			cell_state(z, h, rows, cols, xi, xj, eps, sx, nbx);
+			if (!sx.wet || ((sx.elim >> opp) & 1u)) continue;
+			const double lx = lam1[nbx[0]];
+			double sg = 0.0;
+#pragma unroll
+			for (int q = 1; q < 5; ++q)
+				if (!((sx.elim >> q) & 1u)) sg += lam1[nbx[q]] - lx;
+			const double Bx = sx.cnt ? r * sg / sx.cnt : 0.0;
+			const double du = -r * (lc - lx) + Bx;
+			acc += du;
+			lz += du;
+		}
+		lam0[c] = acc;
+		if (i >= box.r0 && i < box.r0 + box.nr && j >= box.c0 && j < box.c0 + box.nc) lam_z[c] += lz;
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		dr += __shfl_down(dr, off, 64);
+		de += __shfl_down(de, off, 64);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		red_r[threadIdx.x >> 6] = dr;
+		red_e[threadIdx.x >> 6] = de;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double a = 0.0, b = 0.0;
+		for (int w = 0; w < ADJ_BLOCK / 64; ++w) {
+			a += red_r[w];
+			b += red_e[w];
+		}
+		partial[2 * blockIdx.x] += a;
+		partial[2 * blockIdx.x + 1] += b;
+	}
+}
+
+// One forward step with the fused sweep (toroidal, any column count).
+__global__ __launch_bounds__(64 * FUSED_WAVES) void fwd_step(const double* __restrict__ z,
+                                                               const double* __restrict__ h,
+                                                               double* __restrict__ hn, double* __restrict__ trash,
+                                                               int rows, int cols, int n_strips, int n_logical,
+                                                               int per_xcd, double r, double eps)
+{
+	const int L = dev::xcd_logical_block(blockIdx.x, per_xcd);
+	if (L >= n_logical) return;
+	const int n_sgroups = (n_strips + FUSED_WAVES - 1) / FUSED_WAVES;
+	const int strip = (L % n_sgroups) * FUSED_WAVES + (threadIdx.x >> 6);
+	const int chunk = L / n_sgroups;
+	if (strip >= n_strips) return;
+	const int r0 = chunk * ADJ_RPW;
+	if (r0 >= rows) return;
+	if (cols % 2 == 0)
+		sweep_strip<1, true>(z, h, hn, trash, rows, cols, strip, r0, min(r0 + ADJ_RPW, rows), r, eps);
+	else
+		sweep_strip<1, false>(z, h, hn, trash, rows, cols, strip, r0, min(r0 + ADJ_RPW, rows), r, eps);
+}
+
+struct Fwd {
+	int n_strips, n_logical, per_xcd, blocks;
+	Fwd(int rows, int cols)
+	{
+		n_strips = (cols + STRIP2_OUT - 1) / STRIP2_OUT;
+		int sgroups = (n_strips + FUSED_WAVES - 1) / FUSED_WAVES;
+		n_logical = sgroups * ((rows + ADJ_RPW - 1) / ADJ_RPW);
+		per_xcd = (n_logical + 7) / 8;
+		blocks = per_xcd * 8;
+	}
+};
+
+struct Scoped {
+	void* p = nullptr;
+	~Scoped()
+	{
+		if (p) (void)hipFree(p);
+	}
+};
+
+int launch_fwd(hipStream_t st, const Fwd& g, const double* z, const double* h, double* hn, double* trash, int rows,
+               int cols, double r, double eps)
+{
+	hipLaunchKernelGGL(fwd_step, dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, st, z, h, hn, trash, rows, cols,
+	                   g.n_strips, g.n_logical, g.per_xcd, r, eps);
+	return 0;
+}
+
+bool check_args(int rows, int cols, int steps, int ck)
+{
+	if (rows < 3 || cols < 3 || steps < 0 || ck <= 0) {
+		set_error(50, "SciddicaT adjoint: grid must be at least 3 x 3, steps >= 0, ck > 0");
+		return false;
+	}
+	return true;
+}
+
+}  // namespace
+
+}  // namespace calhip
 
 using namespace calhip;
 
-extern "C" int calhipSciddicaTForwardDev(int, int, const double*, const double*, double, double, int, int, double*,
-                                         double*, double*)
+// Checkpoint k holds h after k*ck steps, k = 0 .. steps/ck (h_out gets h_T).
+extern "C" int calhipSciddicaTForwardDev(int rows, int cols, const double* d_z, const double* d_h0, double p_r,
+                                         double p_eps, int steps, int ck, double* d_ckpt, double* d_h_out,
+                                         double* timing_ms)
 {
-	set_error(91, "calhipSciddicaTForwardDev: not available in this build");
-	return -1;
+	if (!check_args(rows, cols, steps, ck)) return -1;
+	const size_t n = (size_t)rows * cols, bytes = n * sizeof(double);
+	hipStream_t st = nullptr;
+	if (CALHIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking))) return -1;
+	Scoped bufA, bufB, trash;
+	int rc = -1;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	do {
+		if (CALHIP_CHECK(hipMalloc(&bufA.p, bytes)) || CALHIP_CHECK(hipMalloc(&bufB.p, bytes)) ||
+		    CALHIP_CHECK(hipMalloc(&trash.p, 128 * sizeof(double))))
+			break;
+		if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
+		Fwd g(rows, cols);
+		double* b[2] = {(double*)bufA.p, (double*)bufB.p};
+		if (CALHIP_CHECK(hipMemcpyAsync(b[0], d_h0, bytes, hipMemcpyDeviceToDevice, st))) break;
+		if (CALHIP_CHECK(hipEventRecord(e0, st))) break;
+		for (int t = 0; t < steps; ++t) {
+			if (t % ck == 0 && d_ckpt)
+				(void)hipMemcpyAsync(d_ckpt + (size_t)(t / ck) * n, b[t & 1], bytes, hipMemcpyDeviceToDevice, st);
+			launch_fwd(st, g, d_z, b[t & 1], b[(t + 1) & 1], (double*)trash.p, rows, cols, p_r, p_eps);
+		}
+		if (steps % ck == 0 && d_ckpt)
+			(void)hipMemcpyAsync(d_ckpt + (size_t)(steps / ck) * n, b[steps & 1], bytes, hipMemcpyDeviceToDevice, st);
+		if (CALHIP_CHECK(hipGetLastError())) break;
+		if (CALHIP_CHECK(hipEventRecord(e1, st))) break;
+		if (CALHIP_CHECK(hipMemcpyAsync(d_h_out, b[steps & 1], bytes, hipMemcpyDeviceToDevice, st))) break;
+		if (CALHIP_CHECK(hipStreamSynchronize(st))) break;
+		float ms = 0.f;
+		if (CALHIP_CHECK(hipEventElapsedTime(&ms, e0, e1))) break;
+		if (timing_ms) *timing_ms = ms;
+		rc = 0;
+	} while (0);
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	(void)hipStreamSynchronize(st);
+	(void)hipStreamDestroy(st);
+	return rc;
 }
 
-extern "C" int calhipSciddicaTAdjointDev(int, int, const double*, const double*, double, double, int, int, double*,
-                                         double*, double*, double*, double*)
+// d_lam: dL/dh_T in, dL/dh_0 out.  d_lam_z: dL/dz accumulated (+=) for every
+// cell when lz_box is NULL.  Checkpoints as written by the forward.
+static int adjoint_impl(int rows, int cols, const double* d_z, const double* d_ckpt, double p_r, double p_eps,
+                        int steps, int ck, double* d_lam, double* d_lam_z, double* d_r, double* d_eps,
+                        double* timing_ms, Box box)
 {
-	set_error(92, "calhipSciddicaTAdjointDev: not available in this build");
-	return -1;
+	if (!check_args(rows, cols, steps, ck)) return -1;
+	const size_t n = (size_t)rows * cols, bytes = n * sizeof(double);
+	const int nblk = (int)((n + ADJ_BLOCK - 1) / ADJ_BLOCK);
+	hipStream_t st = nullptr;
+	if (CALHIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking))) return -1;
+	Scoped stash, lamB, part, trash;
+	int rc = -1;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	do {
+		const int seg_len = ck < steps ? ck : (steps > 0 ? steps : 1);
+		if (CALHIP_CHECK(hipMalloc(&stash.p, bytes * (size_t)(seg_len + 1))) ||
+		    CALHIP_CHECK(hipMalloc(&lamB.p, bytes)) ||
+		    CALHIP_CHECK(hipMalloc(&part.p, sizeof(double) * 2 * (size_t)nblk)) ||
+		    CALHIP_CHECK(hipMalloc(&trash.p, 128 * sizeof(double))))
+			break;
+		if (CALHIP_CHECK(hipMemsetAsync(part.p, 0, sizeof(double) * 2 * (size_t)nblk, st))) break;
+		if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
+		if (CALHIP_CHECK(hipEventRecord(e0, st))) break;
+		Fwd g(rows, cols);
+		double* lam[2] = {d_lam, (double*)lamB.p};
+		int cur = 0;   // lam[cur] holds dL/dh at the current time
+		double* S = (double*)stash.p;
+		const int nseg = (steps + ck - 1) / ck;
+		for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+			const int t0 = sgi * ck, t1 = t0 + ck < steps ? t0 + ck : steps;
+			// recompute h_t0 .. h_{t1-1} into the stash
+			if (CALHIP_CHECK(hipMemcpyAsync(S, d_ckpt + (size_t)sgi * n, bytes, hipMemcpyDeviceToDevice, st))) break;
+			for (int t = t0; t < t1 - 1; ++t)
+				launch_fwd(st, g, d_z, S + (size_t)(t - t0) * n, S + (size_t)(t - t0 + 1) * n, (double*)trash.p, rows,
+				           cols, p_r, p_eps);
+			for (int t = t1 - 1; t >= t0; --t) {
+				hipLaunchKernelGGL(adjoint_step, dim3(nblk), dim3(ADJ_BLOCK), 0, st, d_z,
+				                   (const double*)(S + (size_t)(t - t0) * n), (const double*)lam[cur], lam[cur ^ 1],
+				                   d_lam_z, (double*)part.p, rows, cols, p_r, p_eps, box);
+				cur ^= 1;
+			}
+		}
+		if (CALHIP_CHECK(hipGetLastError())) break;
+		if (CALHIP_CHECK(hipEventRecord(e1, st))) break;
+		if (cur == 1 && CALHIP_CHECK(hipMemcpyAsync(d_lam, lam[1], bytes, hipMemcpyDeviceToDevice, st))) break;
+		std::vector<double> hp(2 * (size_t)nblk);
+		if (CALHIP_CHECK(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 2 * (size_t)nblk, hipMemcpyDeviceToHost,
+		                                st)))
+			break;
+		if (CALHIP_CHECK(hipStreamSynchronize(st))) break;
+		double sr = 0.0, se = 0.0;
+		for (int b = 0; b < nblk; ++b) {
+			sr += hp[2 * (size_t)b];
+			se += hp[2 * (size_t)b + 1];
+		}
+		if (d_r) *d_r = sr;
+		if (d_eps) *d_eps = se;
+		float ms = 0.f;
+		if (CALHIP_CHECK(hipEventElapsedTime(&ms, e0, e1))) break;
+		if (timing_ms) *timing_ms = ms;
+		rc = 0;
+	} while (0);
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	(void)hipStreamSynchronize(st);
+	(void)hipStreamDestroy(st);
+	return rc;
+}
+
+extern "C" int calhipSciddicaTAdjointDev(int rows, int cols, const double* d_z, const double* d_ckpt, double p_r,
+                                         double p_eps, int steps, int ck, double* d_lam, double* d_lam_z,
+                                         double* d_r, double* d_eps, double* timing_ms)
+{
+	Box all{0, 0, rows, cols};
+	return adjoint_impl(rows, cols, d_z, d_ckpt, p_r, p_eps, steps, ck, d_lam, d_lam_z, d_r, d_eps, timing_ms, all);
+}
+
+extern "C" int calhipSciddicaTAdjointBoxDev(int rows, int cols, const double* d_z, const double* d_ckpt,
+                                            double p_r, double p_eps, int steps, int ck, double* d_lam,
+                                            double* d_lam_z, double* d_r, double* d_eps, int box_r0, int box_c0,
+                                            int box_rows, int box_cols, double* timing_ms)
+{
+	Box b{box_r0, box_c0, box_rows, box_cols};
+	return adjoint_impl(rows, cols, d_z, d_ckpt, p_r, p_eps, steps, ck, d_lam, d_lam_z, d_r, d_eps, timing_ms, b);
 }

@@ -35,333 +35,13 @@
 
 #include "engine.hpp"
 #include "device_common.hpp"
+#include "sciddicat_device.hpp"
 
 namespace calhip {
 
 namespace {
 
-// avg / cnt exactly as the reference divides (sciddicaT.c:72-73): one IEEE
-// division for every count, which keeps the wave's lanes on one path.
-__device__ __forceinline__ double div_count(double a, int cnt) { return a / (double)cnt; }
-
-// Outflows of a wet cell (h0 > eps) with altitude z0 and neighbour levels
-// u1..u4 (N, W, E, S) = z + h of each neighbour; keep = mask of the
-// non-eliminated neighbours (bits 1..4) for the active-cell variant.
-struct FlowOut {
-	double f0, f1, f2, f3;
-	unsigned keep;
-};
-
-__device__ __forceinline__ FlowOut flows_wet_body(double h0, double z0, double u1, double u2, double u3, double u4,
-                                                  double r, double eps)
-{
-	const double m = h0 - eps;
-	double u[5];
-	u[0] = z0 + eps;
-	u[1] = u1;
-	u[2] = u2;
-	u[3] = u3;
-	u[4] = u4;
-	unsigned elim = 0u;
-	double avg;
-	bool again;
-	do {
-		again = false;
-		avg = m;
-		int cnt = 0;
-#pragma unroll
-		for (int n = 0; n < 5; ++n)
-			if (!((elim >> n) & 1u)) {
-				avg += u[n];
-				cnt++;
-			}
-		if (cnt != 0) avg = div_count(avg, cnt);
-#pragma unroll
-		for (int n = 0; n < 5; ++n)
-			if (avg <= u[n] && !((elim >> n) & 1u)) {
-				elim |= 1u << n;
-				again = true;
-			}
-	} while (again);
-	FlowOut o;
-	o.f0 = (elim & 2u) ? 0.0 : (avg - u[1]) * r;
-	o.f1 = (elim & 4u) ? 0.0 : (avg - u[2]) * r;
-	o.f2 = (elim & 8u) ? 0.0 : (avg - u[3]) * r;
-	o.f3 = (elim & 16u) ? 0.0 : (avg - u[4]) * r;
-	o.keep = (~elim) & 0x1eu;
-	return o;
-}
-
-// Out-of-line copy used by the streaming kernels: wet cells are rare, and
-// keeping the elimination loop out of their unrolled row loop keeps that loop
-// small enough to stay in the instruction cache.
-__device__ __noinline__ FlowOut flows_wet(double h0, double z0, double u1, double u2, double u3, double u4,
-                                          double r, double eps)
-{
-	return flows_wet_body(h0, z0, u1, u2, u3, u4, r, eps);
-}
-
-// Outflows of two cells at once (the two cells of a lane in the two-column
-// kernels): one loop runs both minimisations until both have converged, which
-// halves the code and interleaves two independent dependency chains.
-struct PairFlows {
-	double a[4], b[4];
-};
-
-__device__ __forceinline__ void avg_pass(const double u[5], double m, unsigned& elim, double& avg, bool& again)
-{
-	again = false;
-	avg = m;
-	int cnt = 0;
-#pragma unroll
-	for (int n = 0; n < 5; ++n)
-		if (!((elim >> n) & 1u)) {
-			avg += u[n];
-			cnt++;
-		}
-	if (cnt != 0) avg = div_count(avg, cnt);
-#pragma unroll
-	for (int n = 0; n < 5; ++n)
-		if (avg <= u[n] && !((elim >> n) & 1u)) {
-			elim |= 1u << n;
-			again = true;
-		}
-}
-
-__device__ __forceinline__ void pair_flows(double ha, double za, double ua1, double ua2, double ua3, double ua4,
-                                           double hb, double zb, double ub1, double ub2, double ub3, double ub4,
-                                           double r, double eps, PairFlows& o)
-{
-#pragma unroll
-	for (int q = 0; q < 4; ++q) o.a[q] = o.b[q] = 0.0;
-	bool wa = !(ha <= eps), wb = !(hb <= eps);
-	if (!(wa || wb)) return;
-	const double ma = ha - eps, mb = hb - eps;
-	double u_a[5] = {za + eps, ua1, ua2, ua3, ua4};
-	double u_b[5] = {zb + eps, ub1, ub2, ub3, ub4};
-	unsigned ea = 0u, eb = 0u;
-	double avg_a = 0.0, avg_b = 0.0;
-	bool ga = wa, gb = wb;
-	// A pass after convergence changes nothing (same set, same average, no
-	// new elimination), so both cells run every pass: straight-line code
-	// with two independent chains.
-	while (ga || gb) {
-		bool na, nb;
-		avg_pass(u_a, ma, ea, avg_a, na);
-		avg_pass(u_b, mb, eb, avg_b, nb);
-		ga = ga && na;
-		gb = gb && nb;
-	}
-	if (wa) {
-#pragma unroll
-		for (int n = 1; n < 5; ++n) o.a[n - 1] = ((ea >> n) & 1u) ? 0.0 : (avg_a - u_a[n]) * r;
-	}
-	if (wb) {
-#pragma unroll
-		for (int n = 1; n < 5; ++n) o.b[n - 1] = ((eb >> n) & 1u) ? 0.0 : (avg_b - u_b[n]) * r;
-	}
-}
-
-// Outflows f[0..3] (towards N, W, E, S) of any cell: zero when h0 <= eps
-// (the process returns before writing them; the steering keeps them zero).
-template <bool OUTLINE>
-__device__ __forceinline__ unsigned cell_flows_t(double h0, double z0, double u1, double u2, double u3, double u4,
-                                                 double r, double eps, double f[4])
-{
-	f[0] = f[1] = f[2] = f[3] = 0.0;
-	if (h0 <= eps) return 0u;
-	FlowOut o = OUTLINE ? flows_wet(h0, z0, u1, u2, u3, u4, r, eps)
-	                    : flows_wet_body(h0, z0, u1, u2, u3, u4, r, eps);
-	f[0] = o.f0;
-	f[1] = o.f1;
-	f[2] = o.f2;
-	f[3] = o.f3;
-	return o.keep;
-}
-
-__device__ __forceinline__ unsigned cell_flows(double h0, double z0, double u1, double u2, double u3, double u4,
-                                               double r, double eps, double f[4])
-{
-	return cell_flows_t<false>(h0, z0, u1, u2, u3, u4, r, eps, f);
-}
-
-constexpr int FUSED_WAVES = 4;   // waves per block
-constexpr int STRIP_OUT = 60;    // output columns per wave
-
-// Full-grid fused step (see file header).  Block = FUSED_WAVES waves stacked
-// vertically on one column strip; blocks are remapped so each XCD gets a
-// contiguous band of the grid.
-__global__ __launch_bounds__(64 * FUSED_WAVES) void step_fused(
-    const double* __restrict__ z, const double* __restrict__ h, double* __restrict__ hn, int rows, int cols,
-    int n_strips, int n_logical, int per_xcd, int rpw, double r, double eps)
-{
-	const int L = dev::xcd_logical_block(blockIdx.x, per_xcd);
-	if (L >= n_logical) return;
-	const int strip = L % n_strips;
-	const int group = L / n_strips;
-	const int lane = threadIdx.x & 63;
-	const int chunk = group * FUSED_WAVES + (threadIdx.x >> 6);
-	const int r0 = chunk * rpw;
-	if (r0 >= rows) return;
-	const int r1 = min(r0 + rpw, rows);
-
-	const int jj = strip * STRIP_OUT - 2 + lane;           // unwrapped column
-	const int j = ((jj % cols) + cols) % cols;
-	const bool out_lane = lane >= 2 && lane < 2 + STRIP_OUT && jj < cols;
-
-	// window: rows k-1 (zm,hm), k (zc,hc), k+1 (zs,hs)
-	int rm = dev::wrap1(r0 - 2, rows), rc = dev::wrap1(r0 - 1, rows);
-	double zm = z[(size_t)rm * cols + j], hm = h[(size_t)rm * cols + j];
-	double zc = z[(size_t)rc * cols + j], hc = h[(size_t)rc * cols + j];
-	int rs = dev::wrap1(r0, rows);
-	double zs = z[(size_t)rs * cols + j], hs = h[(size_t)rs * cols + j];
-
-	double fp0 = 0.0, fp1 = 0.0, fp2 = 0.0, fp3 = 0.0;  // f(k-1)
-	double fpp3 = 0.0;                                   // f(k-2).S
-	double hp = 0.0;                                     // h(k-1)
-
-	for (int k = r0 - 1; k <= r1; ++k) {
-		// prefetch row k+2
-		int rn = dev::wrap1(k + 2, rows);
-		double zn = 0.0, hn2 = 0.0;
-		if (k + 2 <= r1 + 1) {
-			zn = z[(size_t)rn * cols + j];
-			hn2 = h[(size_t)rn * cols + j];
-		}
-		const double uc = zc + hc;
-		const double uw = dev::from_left(uc);
-		const double ue = dev::from_right(uc);
-		double f[4];
-		cell_flows(hc, zc, zm + hm, uw, ue, zs + hs, r, eps, f);
-		if (k - 1 >= r0) {   // wave-uniform
-			const double f2w = dev::from_left(fp2);
-			const double f1e = dev::from_right(fp1);
-			double o = hp;
-			o += fpp3 - fp0;
-			o += f2w - fp1;
-			o += f1e - fp2;
-			o += f[0] - fp3;
-			if (out_lane) hn[(size_t)(k - 1) * cols + j] = o;
-		}
-		fpp3 = fp3;
-		fp0 = f[0];
-		fp1 = f[1];
-		fp2 = f[2];
-		fp3 = f[3];
-		hp = hc;
-		zm = zc;
-		hm = hc;
-		zc = zs;
-		hc = hs;
-		zs = zn;
-		hs = hn2;
-	}
-}
-
-// Two columns per lane: 16-byte loads and stores, 128-column strips with 124
-// output columns.  Rows live in a circular register buffer of W = D + 4 rows
-// and the row loop is unrolled W times, so every row keeps its registers for
-// its whole lifetime: the loop has no register copies of in-flight loads and
-// the compiler's vmcnt waits cover only the row being consumed, D rows after
-// its load was issued; row stores stay in flight meanwhile (on gfx9-class
-// GPUs vmcnt counts stores too).  Loads are unconditional (rows past the
-// chunk are clamped to its last halo row) to keep the body straight-line.
-// Needs an even column count (a lane's pair never straddles the wrap).
-constexpr int STRIP2_OUT = 124;
-
-template <int D>
-__global__ __launch_bounds__(64 * FUSED_WAVES) void step_fused2(
-    const double* __restrict__ z, const double* __restrict__ h, double* __restrict__ hn, double* __restrict__ trash,
-    int rows, int cols, int n_strips, int n_logical, int per_xcd, int rpw, double r, double eps)
-{
-	constexpr int W = D + 4;
-	const int L = dev::xcd_logical_block(blockIdx.x, per_xcd);
-	if (L >= n_logical) return;
-	// a block is FUSED_WAVES horizontally adjacent strips of one row chunk (their
-	// halo columns overlap in the CU's cache); each XCD gets a band of chunks
-	const int n_sgroups = (n_strips + FUSED_WAVES - 1) / FUSED_WAVES;
-	const int strip = (L % n_sgroups) * FUSED_WAVES + (threadIdx.x >> 6);
-	const int chunk = L / n_sgroups;
-	if (strip >= n_strips) return;
-	const int lane = threadIdx.x & 63;
-	const int r0 = chunk * rpw;
-	if (r0 >= rows) return;
-	const int r1 = min(r0 + rpw, rows);
-	const int last_row = r1 + 1;   // last halo row needed (unwrapped)
-
-	const int jj = strip * STRIP2_OUT - 2 + 2 * lane;   // unwrapped column of the left element
-	const int j = ((jj % cols) + cols) % cols;           // even, j+1 < cols
-	const bool out_lane = lane >= 1 && lane <= 62 && jj < cols;
-
-	auto ld = [&](const double* a, int row) -> double2 {
-		row = dev::wrap1(min(row, last_row), rows);
-		return *reinterpret_cast<const double2*>(a + (size_t)row * cols + j);
-	};
-
-	// slot of row x: (x - (r0 - 2)) mod W; rows r0-2 .. r0+D preloaded
-	double2 bz[W], bh[W];
-#pragma unroll
-	for (int q = 0; q < W - 1; ++q) {
-		bz[q] = ld(z, r0 - 2 + q);
-		bh[q] = ld(h, r0 - 2 + q);
-		__builtin_amdgcn_sched_barrier(0);   // issue in the order the rows are used
-	}
-	bz[W - 1] = make_double2(0.0, 0.0);
-	bh[W - 1] = make_double2(0.0, 0.0);
-
-	// flows of row k-1 for both elements (a = left, b = right), f(k-2).S
-	double pa0 = 0.0, pa1 = 0.0, pa2 = 0.0, pa3 = 0.0, pb0 = 0.0, pb1 = 0.0, pb2 = 0.0, pb3 = 0.0;
-	double ppa3 = 0.0, ppb3 = 0.0;
-
-	for (int base = r0 - 1; base <= r1; base += W) {
-#pragma unroll
-		for (int d = 0; d < W; ++d) {
-			const int k = base + d;
-			const int sm = d % W, sc = (d + 1) % W, ss = (d + 2) % W, sp = (d + W - 1) % W;
-			// prefetch row k+2+D into the slot of row k-2 (no longer needed)
-			bz[sp] = ld(z, k + 2 + D);
-			bh[sp] = ld(h, k + 2 + D);
-			const double2 zm = bz[sm], hm = bh[sm], zc = bz[sc], hc = bh[sc], zs = bz[ss], hs = bh[ss];
-			const double ua = zc.x + hc.x, ub = zc.y + hc.y;
-			const double uw = dev::from_left(ub);    // west of a: right element of lane-1
-			const double ue = dev::from_right(ua);   // east of b: left element of lane+1
-			PairFlows pf;
-			pair_flows(hc.x, zc.x, zm.x + hm.x, uw, ub, zs.x + hs.x, hc.y, zc.y, zm.y + hm.y, ua, ue, zs.y + hs.y,
-			           r, eps, pf);
-			const double* fa = pf.a;
-			const double* fb = pf.b;
-			const double f2w = dev::from_left(pb2);   // W neighbour of a: b of lane-1, its E outflow
-			const double f1e = dev::from_right(pa1);  // E neighbour of b: a of lane+1, its W outflow
-			double oa = hm.x, ob = hm.y;             // h of row k-1
-			oa += ppa3 - pa0;
-			oa += f2w - pa1;
-			oa += pb1 - pa2;
-			oa += fa[0] - pa3;
-			ob += ppb3 - pb0;
-			ob += pa2 - pb1;
-			ob += f1e - pb2;
-			ob += fb[0] - pb3;
-			// unconditional store (lanes and rows that own no output write to a
-			// per-lane slot of the trash row) keeps the body branch-free
-			const bool st = out_lane && k - 1 >= r0 && k <= r1;
-			double* dst = st ? hn + (size_t)(k - 1) * cols + j : trash + 2 * lane;
-			*reinterpret_cast<double2*>(dst) = make_double2(oa, ob);
-			ppa3 = pa3;
-			ppb3 = pb3;
-			pa0 = fa[0];
-			pa1 = fa[1];
-			pa2 = fa[2];
-			pa3 = fa[3];
-			pb0 = fb[0];
-			pb1 = fb[1];
-			pb2 = fb[2];
-			pb3 = fb[3];
-			// keep each row's prefetch where it is written (the scheduler
-			// would otherwise sink loads towards their use)
-			__builtin_amdgcn_sched_barrier(0);
-		}
-	}
-}
+using namespace sct;
 
 struct FlowPtrs {
 	double* f[4];
@@ -493,19 +173,6 @@ struct FusedGeom {
 	int n_strips, n_groups, n_logical, per_xcd, blocks, rpw;
 };
 
-static FusedGeom fused_geom(int rows, int cols, int rpw, int strip_out)
-{
-	FusedGeom g;
-	g.rpw = rpw;
-	g.n_strips = (cols + strip_out - 1) / strip_out;
-	int chunks = (rows + rpw - 1) / rpw;
-	g.n_groups = (chunks + FUSED_WAVES - 1) / FUSED_WAVES;
-	g.n_logical = g.n_strips * g.n_groups;
-	g.per_xcd = (g.n_logical + 7) / 8;
-	g.blocks = g.per_xcd * 8;
-	return g;
-}
-
 // Two-column kernel: a logical block is FUSED_WAVES adjacent strips of one
 // row chunk.
 static FusedGeom fused_geom2(int rows, int cols, int rpw)
@@ -519,18 +186,6 @@ static FusedGeom fused_geom2(int rows, int cols, int rpw)
 	g.per_xcd = (g.n_logical + 7) / 8;
 	g.blocks = g.per_xcd * 8;
 	return g;
-}
-
-// Rows per wave: the smallest chunk height (>= 16) for which the whole grid is
-// covered by one round of resident blocks (blocks_resident), so no block
-// starts after the first ones finish.
-static int pick_rows_per_wave(int rows, int cols, int strip_out, int blocks_resident)
-{
-	for (int rpw = 16; rpw < 4096; ++rpw) {
-		FusedGeom g = fused_geom(rows, cols, rpw, strip_out);
-		if (g.n_logical <= blocks_resident) return rpw;
-	}
-	return 4096;
 }
 
 // ------------------------------------------------------------------ program
@@ -586,25 +241,11 @@ struct SciddicaTProgram : Program {
 
 	// fused sweep launch: two columns per lane when the column count is even
 	int prefetch = 1;
-	int resident_blocks = 0;
 	double* d_trash = nullptr;   // sink for the stores of halo lanes
 
 	~SciddicaTProgram() override
 	{
 		if (d_trash) hipFree(d_trash);
-	}
-
-	template <typename K>
-	int resident(K kernel)
-	{
-		if (resident_blocks > 0) return resident_blocks;
-		int dev = 0, cus = 0, per_cu = 0;
-		if (CALHIP_CHECK(hipGetDevice(&dev))) return -1;
-		if (CALHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev))) return -1;
-		if (CALHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * FUSED_WAVES, 0)))
-			return -1;
-		resident_blocks = cus * (per_cu > 0 ? per_cu : 1);
-		return resident_blocks;
 	}
 
 	template <int D>
@@ -617,32 +258,24 @@ struct SciddicaTProgram : Program {
 		int rpw = rpw_override > 0 ? rpw_override : 8;
 		FusedGeom g = fused_geom2(rows, cols, rpw);
 		if (!d_trash && CALHIP_CHECK(hipMalloc(&d_trash, 128 * sizeof(double)))) return -1;
-		hipLaunchKernelGGL(step_fused2<D>, dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, m->stream,
-		                   (const double*)z->d_cur, (const double*)h->d_cur, (double*)h->d_nxt, d_trash, rows,
-		                   cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps);
+		if (cols % 2 == 0)
+			hipLaunchKernelGGL((step_fused2<D, true>), dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, m->stream,
+			                   (const double*)z->d_cur, (const double*)h->d_cur, (double*)h->d_nxt, d_trash, rows,
+			                   cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps);
+		else
+			hipLaunchKernelGGL((step_fused2<D, false>), dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, m->stream,
+			                   (const double*)z->d_cur, (const double*)h->d_cur, (double*)h->d_nxt, d_trash, rows,
+			                   cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps);
 		return 0;
 	}
 
 	int launch_fused(ModelX* m, double r, double eps)
 	{
-		const int rows = m->pub.rows, cols = m->pub.columns;
-		if (cols % 2 == 0 && prefetch > 0) {
-			switch (prefetch) {
-			case 1: return launch2<1>(m, r, eps);
-			case 2: return launch2<2>(m, r, eps);
-			case 3: return launch2<3>(m, r, eps);
-			case 6: return launch2<6>(m, r, eps);
-			default: return launch2<4>(m, r, eps);
-			}
+		switch (prefetch) {
+		case 2: return launch2<2>(m, r, eps);
+		case 4: return launch2<4>(m, r, eps);
+		default: return launch2<1>(m, r, eps);
 		}
-		int rb = resident(step_fused);
-		if (rb < 0) return -1;
-		int rpw = rpw_override > 0 ? rpw_override : pick_rows_per_wave(rows, cols, STRIP_OUT, rb);
-		FusedGeom g = fused_geom(rows, cols, rpw, STRIP_OUT);
-		hipLaunchKernelGGL(step_fused, dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, m->stream,
-		                   (const double*)z->d_cur, (const double*)h->d_cur, (double*)h->d_nxt, rows, cols,
-		                   g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps);
-		return 0;
 	}
 
 	int check_entry(ModelX* m)

@@ -152,3 +152,28 @@ def life_run(alive0, steps):
         return m.read()
     finally:
         m.close()
+
+
+def sciddicat_batch(instances, steps):
+    """Batched ensemble through calhipSciddicaTBatchRun.
+
+    instances: list of (z, h, p_r, p_eps) with z, h before the init process.
+    Returns (list of final h, GPU milliseconds of the stepping launches)."""
+    from . import lib
+    n = len(instances)
+    zs = [np.ascontiguousarray(i[0], np.float64) for i in instances]
+    hs = [np.ascontiguousarray(i[1], np.float64) for i in instances]
+    outs = [np.empty_like(h) for h in hs]
+    rows = (C.c_int * n)(*[z.shape[0] for z in zs])
+    cols = (C.c_int * n)(*[z.shape[1] for z in zs])
+    pr = (C.c_double * n)(*[float(i[2]) for i in instances])
+    pe = (C.c_double * n)(*[float(i[3]) for i in instances])
+    zp = (_dp * n)(*[_d(z) for z in zs])
+    hp = (_dp * n)(*[_d(h) for h in hs])
+    op = (_dp * n)(*[_d(o) for o in outs])
+    ms = C.c_double(0.0)
+    L = lib()
+    L.calhipClearError()
+    rc = L.calhipSciddicaTBatchRun(n, rows, cols, pr, pe, zp, hp, int(steps), op, C.byref(ms))
+    check(rc, "calhipSciddicaTBatchRun")
+    return outs, ms.value

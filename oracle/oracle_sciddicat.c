@@ -281,7 +281,9 @@ int orc_sciddicat_adjoint(int rows, int cols, const double* z, const double* h0,
 						A += p_r * g_k;
 						dr += g_k * (cf.avg - cf.u[k]);
 					}
-				B = A / cf.cnt;
+				/* cnt == 0: rounding eliminated every level (h - eps below the
+				 * resolution of the level sum), so no flow depends on avg */
+				B = cf.cnt ? A / cf.cnt : 0.0;
 				lam_prev[c] += B;                       /* m = h - eps */
 				de -= B;
 				if (!cf.elim[0]) {                      /* u0 = z + eps */

@@ -1,0 +1,45 @@
+"""Batched SciddicaT ensemble: every instance equals its own single run
+(bit-exact), against the oracle and against the single-model GPU path;
+uneven sizes including odd column counts and grids narrower than a strip."""
+import numpy as np
+import pytest
+
+import native
+from opencal_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_vs_oracle():
+    from opencal_amd.models import sciddicat_batch
+    specs = synth.batch_instances(12, seed=77, lo=24, hi=180)
+    insts = []
+    for rows, cols, pr, pe, seed in specs:
+        z, h, _ = synth.sciddicat_inputs(rows, cols, seed, patch=min(12, rows // 4 or 1))
+        insts.append((z, h, pr, pe))
+    print("case batch sizes", [(r, c) for r, c, *_ in specs], flush=True)
+    outs, ms = sciddicat_batch(insts, 150)
+    assert ms > 0
+    for (z, h, pr, pe), got in zip(insts, outs):
+        _, want = native.oracle_sciddicat(z, h, pr, pe, 150, False)
+        assert np.array_equal(got, want)
+
+
+def test_batch_vs_single_runs():
+    from opencal_amd.models import sciddicat_batch, sciddicat_run
+    specs = [(37, 41, 0.5, 0.001, 1), (130, 257, 0.33, 0.006, 2), (96, 128, 0.69, 0.0002, 3), (5, 3, 0.5, 0.001, 4)]
+    insts = []
+    for rows, cols, pr, pe, seed in specs:
+        z, h, _ = synth.sciddicat_inputs(rows, cols, seed, patch=min(10, rows // 4 or 1))
+        insts.append((z, h, pr, pe))
+    outs, _ = sciddicat_batch(insts, 200)
+    for (z, h, pr, pe), got in zip(insts, outs):
+        _, single, _ = sciddicat_run(z, h, pr, pe, 200, False)
+        assert np.array_equal(got, single)
+
+
+def test_batch_zero_steps_returns_input():
+    from opencal_amd.models import sciddicat_batch
+    z, h, _ = synth.sciddicat_inputs(40, 50, 3, patch=8)
+    outs, _ = sciddicat_batch([(z, h, 0.5, 0.001)], 0)
+    assert np.array_equal(outs[0], h)

@@ -150,3 +150,17 @@ def test_step_linear_in_p_r():
     d1 = native.oracle_step(z, h, 0.1, 0.01) - h
     d2 = native.oracle_step(z, h, 0.2, 0.01) - h
     assert np.allclose(d2, 2 * d1, rtol=1e-9, atol=1e-13)
+
+
+def test_adjoint_oracle_all_levels_eliminated():
+    """h just above eps at ~500 m: rounding makes the level average equal the
+    levels, every level is eliminated (count 0) and the reference moves
+    nothing; the adjoint of that cell is zero, not 0/0."""
+    z = np.full((6, 6), 500.0)
+    h = np.zeros((6, 6))
+    h[2, 2] = 0.001 + 1e-14
+    z[2, 2] -= h[2, 2]
+    assert native.oracle_step(z, h, 0.5, 0.001)[2, 2] == h[2, 2]
+    lh, lz, dr, de = native.oracle_adjoint(z, h, 0.5, 0.001, 3, np.ones((6, 6)))
+    assert np.isfinite(lh).all() and np.isfinite(lz).all()
+    assert np.array_equal(lh, np.ones((6, 6))) and dr == 0.0 and de == 0.0
