@@ -158,9 +158,9 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) void fwd_step(const double* __res
 	const int r0 = chunk * ADJ_RPW;
 	if (r0 >= rows) return;
 	if (cols % 2 == 0)
-		sweep_strip<1, true>(z, h, hn, trash, rows, cols, strip, r0, min(r0 + ADJ_RPW, rows), r, eps);
+		sweep_chunk<1, true>(z, h, hn, trash, rows, cols, strip, chunk, r0, min(r0 + ADJ_RPW, rows), r, eps);
 	else
-		sweep_strip<1, false>(z, h, hn, trash, rows, cols, strip, r0, min(r0 + ADJ_RPW, rows), r, eps);
+		sweep_chunk<1, false>(z, h, hn, trash, rows, cols, strip, chunk, r0, min(r0 + ADJ_RPW, rows), r, eps);
 }
 
 struct Fwd {

@@ -47,9 +47,10 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) void batch_step(const BatchInst* 
 	const int r0 = it.z * BATCH_RPW;
 	const int r1 = min(r0 + BATCH_RPW, I.rows);
 	if (I.cols % 2 == 0)
-		sweep_strip<1, true>(z + I.off, h + I.off, hn + I.off, trash, I.rows, I.cols, strip, r0, r1, I.r, I.eps);
+		sweep_chunk<1, true>(z + I.off, h + I.off, hn + I.off, trash, I.rows, I.cols, strip, it.z, r0, r1, I.r, I.eps);
 	else
-		sweep_strip<1, false>(z + I.off, h + I.off, hn + I.off, trash, I.rows, I.cols, strip, r0, r1, I.r, I.eps);
+		sweep_chunk<1, false>(z + I.off, h + I.off, hn + I.off, trash, I.rows, I.cols, strip, it.z, r0, r1, I.r,
+		                      I.eps);
 }
 
 struct DevBuf {

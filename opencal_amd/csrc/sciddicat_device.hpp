@@ -153,75 +153,91 @@ constexpr int STRIP2_OUT = 124;
 // One wave sweeps rows [r0, r1) of the 128-column strip `strip` of a
 // rows x cols toroidal grid.  EVEN: cols is even, so a lane's two cells are
 // one aligned 16-byte access; otherwise each cell is loaded and stored on its
-// own (the pair may straddle the wrap).
-template <int D, bool EVEN>
+// own (the pair may straddle the wrap).  UP: walk the chunk bottom-up.
+// Neighbouring chunks walk in opposite directions, so the halo rows they share
+// are loaded by both at about the same time and the second load hits L2.
+//
+// Rows are consumed as a stream: position s is row r0-2+s walking down,
+// r1+1-s walking up; iteration q computes the flows of position q+1 from
+// positions q..q+2 and writes row(q) for q >= 2 (its flows, those of the row
+// behind it, q-1, and those of the row ahead of it, q+1, are then known).
+template <int D, bool EVEN, bool UP>
 __device__ __forceinline__ void sweep_strip(const double* __restrict__ z, const double* __restrict__ h,
                                             double* __restrict__ hn, double* __restrict__ trash, int rows, int cols,
                                             int strip, int r0, int r1, double r, double eps)
 {
 	constexpr int W = D + 4;
 	const int lane = threadIdx.x & 63;
-	const int last_row = r1 + 1;   // last halo row needed (unwrapped)
+	const int n_it = r1 - r0 + 2;   // flows computed for rows r0-1 .. r1
+	const int last_pos = n_it + 1;
 	const int jj = strip * STRIP2_OUT - 2 + 2 * lane;   // unwrapped column of the left element
 	const int j = ((jj % cols) + cols) % cols;
 	const int j2 = j + 1 == cols ? 0 : j + 1;
 	const bool out_a = lane >= 1 && lane <= 62 && jj < cols;
 	const bool out_b = lane >= 1 && lane <= 62 && jj + 1 < cols;
 
-	auto ld = [&](const double* a, int row) -> double2 {
-		row = dev::wrap1(min(row, last_row), rows);
+	auto row_of = [&](int pos) -> int { return UP ? r1 + 1 - pos : r0 - 2 + pos; };
+	auto ld = [&](const double* a, int pos) -> double2 {
+		const int row = dev::wrap1(row_of(min(pos, last_pos)), rows);
 		const double* p = a + (size_t)row * cols;
 		if (EVEN) return *reinterpret_cast<const double2*>(p + j);
 		return make_double2(p[j], p[j2]);
 	};
 
-	// slot of row x: (x - (r0 - 2)) mod W; rows r0-2 .. r0+D preloaded
+	// slot of stream position s: s mod W; positions 0 .. W-2 preloaded
 	double2 bz[W], bh[W];
 #pragma unroll
 	for (int q = 0; q < W - 1; ++q) {
-		bz[q] = ld(z, r0 - 2 + q);
-		bh[q] = ld(h, r0 - 2 + q);
+		bz[q] = ld(z, q);
+		bh[q] = ld(h, q);
 		__builtin_amdgcn_sched_barrier(0);   // issue in the order the rows are used
 	}
 	bz[W - 1] = make_double2(0.0, 0.0);
 	bh[W - 1] = make_double2(0.0, 0.0);
 
-	// flows of row k-1 for both elements (a = left, b = right), f(k-2).S
+	// flows of the row behind (position q) for both elements (a = left,
+	// b = right), and the one component of position q-1 that row(q) needs:
+	// the outflow towards row(q) (S outflow walking down, N outflow walking up)
 	double pa0 = 0.0, pa1 = 0.0, pa2 = 0.0, pa3 = 0.0, pb0 = 0.0, pb1 = 0.0, pb2 = 0.0, pb3 = 0.0;
-	double ppa3 = 0.0, ppb3 = 0.0;
+	double ppa = 0.0, ppb = 0.0;
 
-	for (int base = r0 - 1; base <= r1; base += W) {
+	for (int base = 0; base < n_it; base += W) {
 #pragma unroll
 		for (int d = 0; d < W; ++d) {
-			const int k = base + d;
-			const int sm = d % W, sc = (d + 1) % W, ss = (d + 2) % W, sp = (d + W - 1) % W;
-			// prefetch row k+2+D into the slot of row k-2 (no longer needed)
-			bz[sp] = ld(z, k + 2 + D);
-			bh[sp] = ld(h, k + 2 + D);
-			const double2 zm = bz[sm], hm = bh[sm], zc = bz[sc], hc = bh[sc], zs = bz[ss], hs = bh[ss];
+			const int q = base + d;
+			const int sb = d % W, sc = (d + 1) % W, sa = (d + 2) % W, sp = (d + W - 1) % W;
+			// prefetch position q+W-1 into the slot of position q-1 (no longer needed)
+			bz[sp] = ld(z, q + W - 1);
+			bh[sp] = ld(h, q + W - 1);
+			// behind / current / ahead rows; north is behind when walking down
+			const double2 zb = bz[sb], hb = bh[sb], zc = bz[sc], hc = bh[sc], za = bz[sa], ha = bh[sa];
+			const double2 zN = UP ? za : zb, hN = UP ? ha : hb, zS = UP ? zb : za, hS = UP ? hb : ha;
 			const double ua = zc.x + hc.x, ub = zc.y + hc.y;
 			const double uw = dev::from_left(ub);    // west of a: right element of lane-1
 			const double ue = dev::from_right(ua);   // east of b: left element of lane+1
 			PairFlows pf;
-			pair_flows(hc.x, zc.x, zm.x + hm.x, uw, ub, zs.x + hs.x, hc.y, zc.y, zm.y + hm.y, ua, ue, zs.y + hs.y,
+			pair_flows(hc.x, zc.x, zN.x + hN.x, uw, ub, zS.x + hS.x, hc.y, zc.y, zN.y + hN.y, ua, ue, zS.y + hS.y,
 			           r, eps, pf);
 			const double* fa = pf.a;
 			const double* fb = pf.b;
+			// row(q) = the row behind: its north / south neighbour terms
+			const double nA = UP ? fa[3] : ppa, sA = UP ? ppa : fa[0];
+			const double nB = UP ? fb[3] : ppb, sB = UP ? ppb : fb[0];
 			const double f2w = dev::from_left(pb2);   // W neighbour of a: b of lane-1, its E outflow
 			const double f1e = dev::from_right(pa1);  // E neighbour of b: a of lane+1, its W outflow
-			double oa = hm.x, ob = hm.y;             // h of row k-1
-			oa += ppa3 - pa0;
+			double oa = hb.x, ob = hb.y;             // h of row(q)
+			oa += nA - pa0;
 			oa += f2w - pa1;
 			oa += pb1 - pa2;
-			oa += fa[0] - pa3;
-			ob += ppb3 - pb0;
+			oa += sA - pa3;
+			ob += nB - pb0;
 			ob += pa2 - pb1;
 			ob += f1e - pb2;
-			ob += fb[0] - pb3;
+			ob += sB - pb3;
 			// unconditional stores (lanes and rows that own no output write to
 			// a per-lane slot of the trash row) keep the body branch-free
-			const bool row_ok = k - 1 >= r0 && k <= r1;
-			double* orow = hn + (size_t)(k - 1) * cols;
+			const bool row_ok = q >= 2 && q < n_it;
+			double* orow = hn + (size_t)row_of(q) * cols;
 			if (EVEN) {
 				double* dst = (row_ok && out_a) ? orow + j : trash + 2 * lane;
 				*reinterpret_cast<double2*>(dst) = make_double2(oa, ob);
@@ -229,8 +245,8 @@ __device__ __forceinline__ void sweep_strip(const double* __restrict__ z, const 
 				*((row_ok && out_a) ? orow + j : trash + 2 * lane) = oa;
 				*((row_ok && out_b) ? orow + j2 : trash + 2 * lane + 1) = ob;
 			}
-			ppa3 = pa3;
-			ppb3 = pb3;
+			ppa = UP ? pa0 : pa3;
+			ppb = UP ? pb0 : pb3;
 			pa0 = fa[0];
 			pa1 = fa[1];
 			pa2 = fa[2];
@@ -244,6 +260,18 @@ __device__ __forceinline__ void sweep_strip(const double* __restrict__ z, const 
 			__builtin_amdgcn_sched_barrier(0);
 		}
 	}
+}
+
+// Both walking directions; chunks alternate.
+template <int D, bool EVEN>
+__device__ __forceinline__ void sweep_chunk(const double* __restrict__ z, const double* __restrict__ h,
+                                            double* __restrict__ hn, double* __restrict__ trash, int rows, int cols,
+                                            int strip, int chunk, int r0, int r1, double r, double eps)
+{
+	if (chunk & 1)
+		sweep_strip<D, EVEN, true>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
+	else
+		sweep_strip<D, EVEN, false>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
 }
 
 template <int D, bool EVEN>
@@ -261,7 +289,7 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) void step_fused2(
 	if (strip >= n_strips) return;
 	const int r0 = chunk * rpw;
 	if (r0 >= rows) return;
-	sweep_strip<D, EVEN>(z, h, hn, trash, rows, cols, strip, r0, min(r0 + rpw, rows), r, eps);
+	sweep_chunk<D, EVEN>(z, h, hn, trash, rows, cols, strip, chunk, r0, min(r0 + rpw, rows), r, eps);
 }
 
 }  // namespace sct
