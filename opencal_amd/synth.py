@@ -107,3 +107,37 @@ def read_matrix_gz(path: str) -> np.ndarray:
     with gzip.open(path, "rt") as f:
         rows = [np.array(line.split(), dtype=np.float64) for line in f if line.strip()]
     return np.vstack(rows)
+
+
+# SCIARA-fv2 parameters in the order of the reference configuration file
+# (OpenCAL-OMP/examples/calomp_sciara-fv2/data/2006/2006_000000000000.cfg):
+# Pclock [s], PTsol [K], PTvent [K], Pr(Tsol), Pr(Tvent), Phc(Tsol) [m],
+# Phc(Tvent) [m], Pcool, Prho [kg/m3], Pepsilon, Psigma [W/m2K4], Pcv [J/kgK].
+# PTsol is 1100 K here (the sample file has 1143 K).
+SCIARA_PARAMS = (60.0, 1100.0, 1360.0, 0.075, 0.9, 60.0, 0.4, 9.0, 2600.0, 0.9, 5.68e-8, 1150.0)
+
+
+def sciara_inputs(rows: int, cols: int, seed: int, n_vents: int = 3, n_rates: int = 10,
+                  steps_per_rate: int = 1000, vol_lo: float = 5.0, vol_hi: float = 20.0,
+                  cell_size: float = 10.0, params=SCIARA_PARAMS):
+    """Inputs of a SCIARA-fv2 run: DEM of 2000 m base + 500 m slope + 20 bumps,
+    `n_vents` vents at seeded cells in the upslope quarter (kept 8 cells off the
+    border), each emitting a seeded volume U(vol_lo, vol_hi) m3 per step during
+    each of `n_rates` intervals of `steps_per_rate` steps.  The schedule is
+    given, as the reference reads it, in m3/s (volume / Pclock), with
+    emission_time = steps_per_rate * Pclock seconds; the stop condition then
+    ends the run after n_rates * steps_per_rate steps.
+    Returns a dict with z0, cell_size, params, vents (n_vents x 2 int32,
+    row/col), rates (n_vents x n_rates), emission_time."""
+    s = Stream(seed ^ 0x5C1A4A)
+    z0 = dem(rows, cols, seed, base=2000.0)
+    m = 8
+    r_hi = max(m + 1, rows // 4)
+    vr = (m + np.floor(s.take(n_vents) * (r_hi - m))).astype(np.int32)
+    vc = (m + np.floor(s.take(n_vents) * max(cols - 2 * m, 1))).astype(np.int32)
+    vents = np.ascontiguousarray(np.stack([vr, vc], axis=1).astype(np.int32))
+    pclock = float(params[0])
+    vol = s.between(vol_lo, vol_hi, n_vents * n_rates).reshape(n_vents, n_rates)
+    rates = np.ascontiguousarray(vol / pclock)
+    return dict(z0=z0, cell_size=float(cell_size), params=np.array(params, dtype=np.float64),
+                vents=vents, rates=rates, emission_time=int(steps_per_rate * pclock))

@@ -40,6 +40,10 @@ int orc_sciara_run(int rows, int cols, const double* z0, double cell_size,
                    double* out_z, double* out_h, double* out_T, double* out_Msl,
                    int* out_steps, int* out_active);
 
+/* The portable elementary functions elementwise (fn: 0 pow10, 1 atan, 2 cos,
+ * 3 cbrt, 4 log, 5 exp); returns -1 for an unknown fn. */
+int orc_pm_eval(int fn, const double* x, int n, double* y);
+
 /* One SciddicaT full-grid step h -> h_next (flows with the reference's
  * minimisation + width update), used by the adjoint oracle. */
 void orc_sciddicat_step(int rows, int cols, const double* z, const double* h,

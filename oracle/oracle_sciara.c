@@ -1,6 +1,217 @@
-/* ORACLE — test infrastructure, not product code.  SCIARA-fv2 restatement. */
+/* ORACLE — test infrastructure, not product code.
+ * SCIARA-fv2 restated from OpenCAL-OMP/examples/calomp_sciara-fv2/source/Sciara.cpp
+ * (updateVentsEmission :3-16, outflowsMin/empiricalFlows :18-105, width_update
+ * :106-135, updateTemperature :137-169, removeActiveCells :171-175, stopCondition
+ * :177-183, steering :185-197, MakeBorder/simulationInitialize :199-285,
+ * initSciara :287-356), vent thickness from include/vent.h:74-82, run by the
+ * engine loop of OpenCAL-OMP/source/cal2DRun.c:125-139 and cal2D.c:717-776 on a
+ * Moore, non-toroidal (CAL_SPACE_FLAT) model.
+ *
+ * math_mode 0 calls the host libm (pow, atan, cos) as the reference does;
+ * math_mode 1 calls the portable functions that the device program also uses
+ * (opencal_amd/csrc/portable_math.h).
+ *
+ * Out-of-grid neighbours: the reference reads outside its buffers there (no
+ * bounds check on CAL_SPACE_FLAT, cal2D.c:859-870).  Only border cells have
+ * such neighbours; their values are overwritten by the steering (h = T = 0 on
+ * the border mask) before any other cell reads them, so the oracle reads 0.
+ * Pinned by: the reference build in oracle/_ref (bit-exact in math_mode 0),
+ * lava volume balance, temperature bounds. */
 #include "oracle.h"
-#include "portable_math.h"
+#include "oracle_engine.h"
+#include "../opencal_amd/csrc/portable_math.h"
+#include <math.h>
+#include <omp.h>
+
+#define NF 8
+enum { SZ = 0, SLT = 1, ST = 2, F0 = 3 };   /* registration order: Sz, Slt, St, f[0..7] */
+
+typedef struct {
+	orc_grid g;
+	orc_sub_r q[3 + NF];
+	unsigned char* Mb;
+	double* Msl;
+	const double* prm;
+	double Pclock, PTsol, PTvent, Pc, Pac, a, b, c, d, Pcool, Prho, Peps, Psigma, Pcv, rad2;
+	double elapsed, effusion;
+	unsigned emission_time;
+	int n_vents, n_rates;
+	const int* vent_rc;
+	const double* rates;
+	int math;
+} sciara_t;
+
+static double m_pow10(const sciara_t* s, double x) { return s->math ? pm_pow10(x) : pow(10, x); }
+static double m_atan(const sciara_t* s, double x) { return s->math ? pm_atan(x) : atan(x); }
+static double m_cos(const sciara_t* s, double x) { return s->math ? pm_cos(x) : cos(x); }
+static double m_cube(const sciara_t* s, double x) { return s->math ? x * x * x : pow(x, 3.0); }
+static double m_cbrt(const sciara_t* s, double x) { return s->math ? pm_cbrt_pos(x) : pow(x, 1.0 / 3.0); }
+
+/* neighbour n of (i, j) on the flat grid, -1 outside */
+static inline int nb_flat(const orc_grid* g, int i, int j, int n)
+{
+	int ii = i + ORC_DI[n], jj = j + ORC_DJ[n];
+	if (ii < 0 || ii >= g->rows || jj < 0 || jj >= g->cols) return -1;
+	return ii * g->cols + jj;
+}
+
+static inline double getx(const double* a, int c) { return c < 0 ? 0.0 : a[c]; }
+
+/* TVent::thickness (vent.h:74-82) */
+static double vent_thickness(const sciara_t* s, int v)
+{
+	unsigned i = (unsigned)(s->elapsed / s->emission_time);
+	if (i >= (unsigned)s->n_rates) return 0;
+	return s->rates[(size_t)v * s->n_rates + i] / s->Pac * s->Pclock;
+}
+
+static void ep_vents(sciara_t* s, int i, int j)
+{
+	int k;
+	for (k = 0; k < s->n_vents; k++)
+		if (i == s->vent_rc[2 * k] && j == s->vent_rc[2 * k + 1]) {
+			double e = vent_thickness(s, k);
+			int c = i * s->g.cols + j;
+			if (e > 0) {
+				s->q[SLT].nxt[c] = s->q[SLT].cur[c] + e;
+				s->q[ST].nxt[c] = s->PTvent;
+			}
+		}
+}
+
+static void ep_flows(sciara_t* s, int i, int j)
+{
+	const orc_grid* g = &s->g;
+	int c = i * g->cols + j, k, counter, loop;
+	int nbs[9];
+	double z[9], h[9], H[9], theta[9], f[9], avg, t, Pr, hc, sz0;
+	unsigned char keep[9];
+	if (!(s->q[SLT].cur[c] > 0)) return;
+	t = s->q[ST].cur[c];
+	Pr = m_pow10(s, s->a + s->b * t);
+	hc = m_pow10(s, s->c + s->d * t);
+	sz0 = s->q[SZ].cur[c];
+	for (k = 0; k < 9; k++) {
+		nbs[k] = nb_flat(g, i, j, k);
+		h[k] = getx(s->q[SLT].cur, nbs[k]);
+		H[k] = f[k] = theta[k] = 0;
+		if (k < 5)
+			z[k] = getx(s->q[SZ].cur, nbs[k]);
+		else
+			z[k] = sz0 - (sz0 - getx(s->q[SZ].cur, nbs[k])) / s->rad2;
+	}
+	H[0] = z[0];
+	keep[0] = 1;
+	for (k = 1; k < 9; k++)
+		if (z[0] + h[0] > z[k] + h[k]) {
+			H[k] = z[k] + h[k];
+			theta[k] = m_atan(s, ((z[0] + h[0]) - (z[k] + h[k])) / s->Pc);
+			keep[k] = 1;
+		} else
+			keep[k] = 0;
+	do {
+		loop = 0;
+		avg = h[0];
+		counter = 0;
+		for (k = 0; k < 9; k++)
+			if (keep[k]) {
+				avg += H[k];
+				counter++;
+			}
+		if (counter != 0) avg = avg / (double)counter;
+		for (k = 0; k < 9; k++)
+			if (keep[k] && avg <= H[k]) {
+				keep[k] = 0;
+				loop = 1;
+			}
+	} while (loop);
+	for (k = 1; k < 9; k++)
+		if (keep[k] && h[0] > hc * m_cos(s, theta[k])) f[k] = Pr * (avg - H[k]);
+	for (k = 1; k < 9; k++)
+		if (f[k] > 0) {
+			s->q[F0 + k - 1].nxt[c] = f[k];
+			if (g->active_mode) orc_add_active(&s->g, nbs[k]);
+		}
+}
+
+static void ep_width(sciara_t* s, int i, int j)
+{
+	static const int out_idx[NF] = {3, 2, 1, 0, 6, 7, 4, 5};
+	const orc_grid* g = &s->g;
+	int c = i * g->cols + j, n;
+	double ih = s->q[SLT].cur[c], it = s->q[ST].cur[c];
+	double residual = ih, h_next = ih, ht = 0, in_sum = 0, out_sum = 0;
+	for (n = 1; n < 9; n++) {
+		int nb = nb_flat(g, i, j, n);
+		double in_flow = getx(s->q[F0 + out_idx[n - 1]].cur, nb);
+		double out_flow = s->q[F0 + n - 1].cur[c];
+		double neigh_t = getx(s->q[ST].cur, nb);
+		ht += in_flow * neigh_t;
+		in_sum += in_flow;
+		out_sum += out_flow;
+	}
+	h_next += in_sum - out_sum;
+	s->q[SLT].nxt[c] = h_next;
+	if (in_sum > 0 || out_sum > 0) {
+		residual -= out_sum;
+		s->q[ST].nxt[c] = (residual * it + ht) / (residual + in_sum);
+	}
+}
+
+static void ep_temperature(sciara_t* s, int i, int j)
+{
+	int c = i * s->g.cols + j;
+	double sh = s->q[SLT].cur[c], st = s->q[ST].cur[c], sz = s->q[SZ].cur[c];
+	if (sh > 0 && !s->Mb[c]) {
+		double h = sh, T = st;
+		if (h != 0) {
+			double nT = T;
+			double aus = 1.0 + (3 * m_cube(s, nT) * s->Peps * s->Psigma * s->Pclock * s->Pcool) /
+			                       (s->Prho * s->Pcv * h * s->Pac);
+			st = nT / m_cbrt(s, aus);
+			s->q[ST].nxt[c] = st;
+		}
+		if (st <= s->PTsol && sh > 0) {
+			s->q[SZ].nxt[c] = sz + sh;
+			s->Msl[c] = s->Msl[c] + sh;
+			s->q[SLT].nxt[c] = 0;
+			s->q[ST].nxt[c] = s->PTsol;
+		} else
+			s->q[SZ].nxt[c] = sz;
+	}
+}
+
+static void ep_remove(sciara_t* s, int i, int j)
+{
+	int c = i * s->g.cols + j;
+	if (s->q[ST].cur[c] <= s->PTsol && !s->Mb[c]) orc_remove_active(&s->g, c);
+}
+
+typedef void (*sep_fn)(sciara_t*, int, int);
+
+static void apply(sciara_t* s, sep_fn ep)
+{
+	orc_grid* g = &s->g;
+	if (g->cells) {
+		int k;
+#pragma omp parallel for
+		for (k = 0; k < g->ncells; k++) ep(s, g->cells[k] / g->cols, g->cells[k] % g->cols);
+	} else {
+		int i;
+#pragma omp parallel for
+		for (i = 0; i < g->rows; i++) {
+			int j;
+			for (j = 0; j < g->cols; j++) ep(s, i, j);
+		}
+	}
+}
+
+/* evaluatePowerLawParams (Sciara.cpp:199-202): host libm in both modes */
+static void power_law(double sol, double vent, double Tsol, double Tvent, double* k1, double* k2)
+{
+	*k2 = (log10(vent) - log10(sol)) / (Tvent - Tsol);
+	*k1 = log10(sol) - *k2 * Tsol;
+}
 
 int orc_sciara_run(int rows, int cols, const double* z0, double cell_size,
                    const double* params, int n_vents, const int* vent_rc,
@@ -9,9 +220,121 @@ int orc_sciara_run(int rows, int cols, const double* z0, double cell_size,
                    double* out_z, double* out_h, double* out_T, double* out_Msl,
                    int* out_steps, int* out_active)
 {
-	(void)rows; (void)cols; (void)z0; (void)cell_size; (void)params; (void)n_vents;
-	(void)vent_rc; (void)n_rates; (void)rates; (void)emission_time; (void)steps;
-	(void)use_active; (void)math_mode; (void)out_z; (void)out_h; (void)out_T;
-	(void)out_Msl; (void)out_steps; (void)out_active;
-	return -1;
+	sciara_t s;
+	size_t n = (size_t)rows * cols, c;
+	int k, i, j, step, n_eps = use_active ? 5 : 4;
+	sep_fn eps_list[5] = {ep_vents, ep_flows, ep_width, ep_temperature, ep_remove};
+	memset(&s, 0, sizeof(s));
+	orc_grid_init(&s.g, rows, cols, use_active);
+	for (k = 0; k < 3 + NF; k++) orc_sub_alloc(&s.q[k], n);
+	s.Mb = (unsigned char*)calloc(n, 1);
+	s.Msl = (double*)calloc(n, sizeof(double));
+	s.math = math_mode;
+	s.Pclock = params[0];
+	s.PTsol = params[1];
+	s.PTvent = params[2];
+	s.Pcool = params[7];
+	s.Prho = params[8];
+	s.Peps = params[9];
+	s.Psigma = params[10];
+	s.Pcv = params[11];
+	s.Pc = cell_size;
+	s.Pac = cell_size * cell_size;
+	s.rad2 = sqrt(2.0);
+	s.n_vents = n_vents;
+	s.vent_rc = vent_rc;
+	s.n_rates = n_rates;
+	s.rates = rates;
+	s.emission_time = emission_time;
+	/* initSciara + loadConfiguration: Sz loaded (current, next); the rest 0 */
+	for (k = 0; k < 3 + NF; k++) {
+		memset(s.q[k].cur, 0, n * sizeof(double));
+		memset(s.q[k].nxt, 0, n * sizeof(double));
+	}
+	memcpy(s.q[SZ].cur, z0, n * sizeof(double));
+	memcpy(s.q[SZ].nxt, z0, n * sizeof(double));
+
+	/* simulationInitialize (init callback) + implicit update */
+	s.elapsed = 0;
+	s.effusion = (double)emission_time * n_rates;
+	for (j = 0; j < cols; j++) {
+		if (s.q[SZ].cur[j] >= 0) { s.Mb[j] = 1; if (use_active) orc_add_active(&s.g, j); }
+	}
+	for (j = 0; j < cols; j++) {
+		c = (size_t)(rows - 1) * cols + j;
+		if (s.q[SZ].cur[c] >= 0) { s.Mb[c] = 1; if (use_active) orc_add_active(&s.g, (int)c); }
+	}
+	for (i = 0; i < rows; i++) {
+		c = (size_t)i * cols;
+		if (s.q[SZ].cur[c] >= 0) { s.Mb[c] = 1; if (use_active) orc_add_active(&s.g, (int)c); }
+	}
+	for (i = 0; i < rows; i++) {
+		c = (size_t)i * cols + cols - 1;
+		if (s.q[SZ].cur[c] >= 0) { s.Mb[c] = 1; if (use_active) orc_add_active(&s.g, (int)c); }
+	}
+	for (i = 1; i < rows - 1; i++)
+		for (j = 1; j < cols - 1; j++) {
+			c = (size_t)i * cols + j;
+			if (s.q[SZ].cur[c] >= 0)
+				for (k = 1; k < 9; k++)
+					if (s.q[SZ].cur[nb_flat(&s.g, i, j, k)] < 0) {
+						s.Mb[c] = 1;
+						if (use_active) orc_add_active(&s.g, (int)c);
+						break;
+					}
+		}
+	power_law(params[3], params[4], s.PTsol, s.PTvent, &s.a, &s.b);
+	power_law(params[5], params[6], s.PTsol, s.PTvent, &s.c, &s.d);
+	if (use_active)
+		for (k = 0; k < n_vents; k++) orc_add_active(&s.g, vent_rc[2 * k] * cols + vent_rc[2 * k + 1]);
+	orc_update(&s.g, s.q, 3 + NF);   /* the init's own calUpdate2D */
+	orc_update(&s.g, s.q, 3 + NF);   /* calRunInitSimulation2D's implicit update */
+
+	step = 1;
+	for (; step <= steps; step++) {
+		int stop;
+		for (k = 0; k < n_eps; k++) {
+			apply(&s, eps_list[k]);
+			orc_update(&s.g, s.q, 3 + NF);
+		}
+		/* steering */
+		for (k = 0; k < NF; k++) orc_init_sub(&s.g, &s.q[F0 + k], 0.0, 1);
+		for (c = 0; c < n; c++)
+			if (s.Mb[c]) {
+				s.q[SLT].nxt[c] = 0;
+				s.q[ST].nxt[c] = 0;
+			}
+		s.elapsed += s.Pclock;
+		orc_update(&s.g, s.q, 3 + NF);
+		stop = s.elapsed >= s.effusion;
+		if (stop) break;
+	}
+	memcpy(out_z, s.q[SZ].cur, n * sizeof(double));
+	memcpy(out_h, s.q[SLT].cur, n * sizeof(double));
+	memcpy(out_T, s.q[ST].cur, n * sizeof(double));
+	memcpy(out_Msl, s.Msl, n * sizeof(double));
+	if (out_steps) *out_steps = step;   /* CALRun2D.step after the loop */
+	if (out_active) *out_active = s.g.ncells;
+	for (k = 0; k < 3 + NF; k++) orc_sub_free(&s.q[k]);
+	free(s.Mb);
+	free(s.Msl);
+	orc_grid_free(&s.g);
+	return 0;
+}
+
+/* Elementwise evaluation of the portable functions (fn: 0 pow10, 1 atan,
+ * 2 cos, 3 cbrt, 4 log, 5 exp), so tests can bound them against libm. */
+int orc_pm_eval(int fn, const double* x, int n, double* y)
+{
+	int k;
+	for (k = 0; k < n; k++) switch (fn) {
+		case 0: y[k] = pm_pow10(x[k]); break;
+		case 1: y[k] = pm_atan(x[k]); break;
+		case 2: y[k] = pm_cos(x[k]); break;
+		case 3: y[k] = pm_cbrt_pos(x[k]); break;
+		case 4: y[k] = pm_log(x[k]); break;
+		case 5: y[k] = pm_exp(x[k]); break;
+		default: return -1;
+	}
+	return 0;
 }

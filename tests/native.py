@@ -130,3 +130,34 @@ def ref_life(a0, steps):
 
 def set_threads(n: int) -> int:
     return oracle().orc_set_threads(n)
+
+
+# ------------------------------------------------------------------- SCIARA
+
+def _sciara_call(fn, inp, steps, active, extra):
+    z0 = np.ascontiguousarray(inp["z0"], np.float64)
+    rows, cols = z0.shape
+    prm = np.ascontiguousarray(inp["params"], np.float64)
+    vents = np.ascontiguousarray(inp["vents"], np.int32)
+    rates = np.ascontiguousarray(inp["rates"], np.float64)
+    out = {k: np.empty_like(z0) for k in ("z", "h", "T", "Msl")}
+    st, na = C.c_int(0), C.c_int(0)
+    fn.argtypes = None
+    rc = fn(rows, cols, dptr(z0), C.c_double(inp["cell_size"]), dptr(prm), vents.shape[0], iptr(vents),
+            rates.shape[1], dptr(rates), C.c_uint(inp["emission_time"]), steps, int(active), *extra,
+            dptr(out["z"]), dptr(out["h"]), dptr(out["T"]), dptr(out["Msl"]), C.byref(st), C.byref(na))
+    assert rc == 0
+    out["steps"], out["active"] = st.value, na.value
+    return out
+
+
+def oracle_sciara(inp, steps, active=True, portable=False):
+    """Oracle SCIARA-fv2 run; portable=True uses the shared portable functions."""
+    return _sciara_call(oracle().orc_sciara_run, inp, steps, active, (int(portable),))
+
+
+def ref_sciara(inp, steps, active=True):
+    lib = ref("sciara")
+    if lib is None:
+        return None
+    return _sciara_call(lib.ref_sciara_run, inp, steps, active, ())
