@@ -79,6 +79,41 @@ int calhipBindSciddicaT2D(struct CALModel2D* ca2D,
 int calhipBindLife2D(struct CALModel2D* ca2D, CALCallbackFunc2D transition,
                      struct CALSubstate2Db* Qb, struct CALSubstate2Di* Qi);
 
+/* --------------------------------------------------------- SCIARA-fv2 */
+
+/* What the SCIARA-fv2 device program needs from a model built as the
+ * reference example builds it (OpenCAL-OMP/examples/calomp_sciara-fv2/
+ * source/Sciara.cpp:287-356: Moore, CAL_SPACE_FLAT; double-layer Sz, Slt, St
+ * and f[0..7]; single-layer Mb and Msl; processes vents, flows, width,
+ * temperature and, with active cells, remove, registered in that order).
+ * The parameter pointers point to the model's own variables and are read at
+ * every step (the init callback may still compute a, b, c, d).  `steering`
+ * is the model's steering callback (Sciara.cpp:185-197): when calRun2D or
+ * calRunCAStep2D would call it, the engine runs its device twin instead,
+ * which also advances *elapsed_time by *Pclock on the host.  Vents and their
+ * emission-rate table are copied at bind time.
+ * Out-of-grid neighbours (the reference reads outside its buffers there,
+ * cal2D.c:859-870) read as altitude 0, thickness 0, temperature 0 and flow 0. */
+struct calhipSciaraBinding {
+	struct CALSubstate2Dr *Sz, *Slt, *St;
+	struct CALSubstate2Dr* f[8];
+	struct CALSubstate2Db* Mb;
+	struct CALSubstate2Dr* Msl;
+	CALCallbackFunc2D vents, flows, width, temperature, remove; /* remove NULL for CAL_NO_OPT */
+	void (*steering)(struct CALModel2D*);
+	const CALreal *Pclock, *PTsol, *PTvent, *Pc, *Pac, *a, *b, *c, *d, *rad2;
+	const CALreal *Pcool, *Prho, *Pepsilon, *Psigma, *Pcv;
+	CALreal* elapsed_time;
+	const unsigned int* emission_time;
+	int n_vents;
+	const int* vent_rc;   /* n_vents (row, column) pairs */
+	int n_rates;
+	const CALreal* rates; /* n_vents x n_rates emission rates [m^3/s], row-major */
+};
+
+/* Attach the SCIARA-fv2 device program (replaces Sciara.cpp:3-197). */
+int calhipBindSciara2D(struct CALModel2D* ca2D, const struct calhipSciaraBinding* b);
+
 /* --------------------------------------------------- batched SciddicaT */
 
 /* Run `steps` full-grid SciddicaT steps (von Neumann, toroidal, flows reset

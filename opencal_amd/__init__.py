@@ -102,6 +102,17 @@ def _declare_examples(E):
     E.life_model.restype = vp
     E.life_model.argtypes = [vp]
     E.life_destroy.argtypes = [vp]
+    E.sciara_create.restype = vp
+    E.sciara_create.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, C.c_int, ip, C.c_int, dp, C.c_uint,
+                                C.c_int, C.c_int, C.c_int]
+    E.sciara_run.argtypes = [vp]
+    E.sciara_read.argtypes = [vp, C.c_int, dp]
+    E.sciara_active_count.argtypes = [vp]
+    E.sciara_elapsed.restype = C.c_double
+    E.sciara_elapsed.argtypes = [vp]
+    E.sciara_model.restype = vp
+    E.sciara_model.argtypes = [vp]
+    E.sciara_destroy.argtypes = [vp]
 
 
 def last_error() -> tuple[int, str]:

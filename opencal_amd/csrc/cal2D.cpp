@@ -449,6 +449,21 @@ extern "C" void calGlobalTransitionFunction2D(struct CALModel2D* m)
 	if (x->prog->step(x) == 0) x->stats.steps++;
 }
 
+namespace calhip {
+int device_transition(ModelX* x, bool with_steering)
+{
+	if (!program_covers(x)) {
+		set_error(2, "calGlobalTransitionFunction2D: the model's elementary processes are not bound "
+		             "to a device program (calhipBind*2D); there is no CPU execution path");
+		return -1;
+	}
+	if (ensure_device(x)) return -1;
+	int e = with_steering ? x->prog->step_and_steer(x) : x->prog->step(x);
+	if (e == 0) x->stats.steps++;
+	return e;
+}
+}  // namespace calhip
+
 // cal2D.c:717-734
 extern "C" void calApplyElementaryProcess2D(struct CALModel2D* m, CALCallbackFunc2D ep)
 {

@@ -39,18 +39,37 @@ extern "C" void calRunInitSimulation2D(struct CALRun2D* r)
 	}
 }
 
-// cal2DRun.c:88-112
+// Runs the steering callback, or its device twin when the bound program
+// provides one for exactly this callback.
+static void run_steering(struct CALRun2D* r)
+{
+	ModelX* x = X(r->ca2D);
+	if (x->prog && x->prog_steering == r->steering) {
+		if (ensure_device(x) == 0) x->prog->steer(x);
+	} else {
+		r->steering(r->ca2D);
+	}
+}
+
+// cal2DRun.c:88-112.  With the model's own transition and a steering that
+// has a device twin, both go to the program in one call.
 extern "C" CALbyte calRunCAStep2D(struct CALRun2D* r)
 {
-	if (r->globalTransition) {
-		r->globalTransition(r->ca2D);
+	ModelX* x = X(r->ca2D);
+	if (!r->globalTransition && r->steering && x->prog && x->prog_steering == r->steering) {
+		device_transition(x, true);
 		if (r->UPDATE_MODE == CAL_UPDATE_IMPLICIT) calUpdate2D(r->ca2D);
 	} else {
-		calGlobalTransitionFunction2D(r->ca2D);
-	}
-	if (r->steering) {
-		r->steering(r->ca2D);
-		if (r->UPDATE_MODE == CAL_UPDATE_IMPLICIT) calUpdate2D(r->ca2D);
+		if (r->globalTransition) {
+			r->globalTransition(r->ca2D);
+			if (r->UPDATE_MODE == CAL_UPDATE_IMPLICIT) calUpdate2D(r->ca2D);
+		} else {
+			calGlobalTransitionFunction2D(r->ca2D);
+		}
+		if (r->steering) {
+			run_steering(r);
+			if (r->UPDATE_MODE == CAL_UPDATE_IMPLICIT) calUpdate2D(r->ca2D);
+		}
 	}
 	if (r->stopCondition && r->stopCondition(r->ca2D)) return CAL_FALSE;
 	return CAL_TRUE;

@@ -74,6 +74,17 @@ struct Program {
 	virtual int before_modify(ModelX*, SubX*) { return 0; }
 	// The model has just been uploaded (host mirrors are the logical state).
 	virtual int after_upload(ModelX*) { return 0; }
+	// Device twin of the model's steering callback (ModelX::prog_steering).
+	virtual int steer(ModelX*) { return -1; }
+	// One transition followed by the steering twin; programs may fuse them.
+	virtual int step_and_steer(ModelX* m)
+	{
+		if (int e = step(m)) return e;
+		return steer(m);
+	}
+	// Called before the device active set is read back: restores the
+	// row-major order of the list when the program keeps it unordered.
+	virtual int sync_active(ModelX*) { return 0; }
 };
 
 struct SubX {
@@ -121,6 +132,7 @@ struct ModelX {
 
 	Program* prog = nullptr;
 	std::vector<CALCallbackFunc2D> prog_eps;  // processes the program covers
+	void (*prog_steering)(CALModel2D*) = nullptr;  // steering with a device twin
 
 	calhipStats stats{};
 	Timer timer;
@@ -146,6 +158,10 @@ int device_fill(ModelX* m, SubX* s, double v, bool both);
 int device_update(ModelX* m);
 int device_rebuild_active(ModelX* m);
 void flip(SubX* s);
+
+// calGlobalTransitionFunction2D on the device, optionally followed by the
+// program's steering twin (cal2D.cpp); 0 on success.
+int device_transition(ModelX* m, bool with_steering);
 
 // ---- kernel launchers (defined in the .hip files) ----
 namespace k {

@@ -221,5 +221,6 @@ extern "C" int calhipBindLife2D(struct CALModel2D* ca2D, CALCallbackFunc2D trans
 	delete m->prog;
 	m->prog = p;
 	m->prog_eps.assign(1, transition);
+	m->prog_steering = nullptr;
 	return 0;
 }

@@ -33,7 +33,9 @@ static int ensure_active_buffers(ModelX* m)
 {
 	if (!m->active()) return 0;
 	size_t n = m->cells();
-	if (!m->d_flags && CALHIP_CHECK(hipMalloc(&m->d_flags, n))) return -1;
+	// rounded up to whole 16-byte words: kernels read flags 16 at a time and
+	// set one with a 32-bit atomic on the word that holds it
+	if (!m->d_flags && CALHIP_CHECK(hipMalloc(&m->d_flags, (n + 15) & ~(size_t)15))) return -1;
 	if (!m->d_cells && CALHIP_CHECK(hipMalloc(&m->d_cells, n * sizeof(int)))) return -1;
 	if (!m->d_count && CALHIP_CHECK(hipMalloc(&m->d_count, 4 * sizeof(int)))) return -1;
 	return 0;
@@ -99,6 +101,7 @@ int ensure_host(ModelX* m)
 	if (m->phase == Phase::Host) return 0;
 	CALModel2D* p = &m->pub;
 	size_t n = m->cells();
+	if (m->prog && m->prog->sync_active(m)) return -1;
 	for (SubX* s : m->subs) {
 		if (s->tstate != Transient::None && s->tstate != Transient::Zero && m->prog) {
 			if (m->prog->materialise(m, s)) return -1;

@@ -1,0 +1,720 @@
+// SCIARA-fv2 lava-flow program: the processes of
+// OpenCAL-OMP/examples/calomp_sciara-fv2/source/Sciara.cpp (vents :3-16,
+// outflowsMin/empiricalFlows :18-105, width_update :106-135,
+// updateTemperature :137-169, removeActiveCells :171-175) and a device twin
+// of its steering (:185-197), on a Moore, CAL_SPACE_FLAT model.
+//
+// One transition is four launches:
+//   sc_vents    emission at the vent cells that are in the active set (host
+//               computes each vent's thickness from the elapsed time exactly
+//               as TVent::thickness does, include/vent.h:74-82)
+//   sc_flows    outflows of every listed lava cell; a receiver whose flag
+//               goes 0 -> 1 is appended to the list (the set after the
+//               rebuild that follows the process in OpenCAL)
+//   sc_width    mass and heat balance, radiative cooling, solidification and
+//               removal, fused (all three read only the cell itself apart
+//               from the neighbours' flows and temperature, which no launch
+//               of this kernel writes); the new temperature goes to a
+//               separate layer because neighbours still read the old one
+//   sc_finish   copies the new temperature back, compacts the survivors into
+//               the next list and, fused with the steering when the run loop
+//               calls it right after the transition, resets the flows at the
+//               survivors and clears lava and temperature on the border mask
+// The active list is kept unordered on the device (every process is
+// independent of the visiting order); the row-major order of OpenCAL's
+// A.cells is restored whenever the host reads the set.  Lists and counters:
+//   count[0] list length, count[1] cells appended by sc_flows,
+//   count[2] survivors written by sc_finish, count[3] finished blocks (the
+//   last block moves count[2] to count[0] and clears the others).
+// Elementary functions are the portable ones of portable_math.h (+ - * /
+// only), so the oracle built with the same functions reproduces every bit;
+// against a glibc build they differ by a few ulp per call.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "engine.hpp"
+#include "device_common.hpp"
+#include "portable_math.h"
+
+namespace calhip {
+
+namespace {
+
+constexpr int NF = 8;
+constexpr int SC_BLOCK = 256;
+constexpr int VENTS_PER_LAUNCH = 16;
+
+struct Flows8 {
+	double* f[NF];
+};
+
+struct SciaraParams {
+	double Pc, Pac, a, b, c, d, rad2, PTsol, PTvent, Pclock, Pcool, Prho, Peps, Psigma, Pcv;
+};
+
+struct VentArgs {
+	int n;
+	int cell[VENTS_PER_LAUNCH];
+	double e[VENTS_PER_LAUNCH];
+};
+
+// Moore neighbourhood in OpenCAL's order: centre, N, W, E, S, NW, SW, SE, NE
+// (calDefineMooreNeighborhood2D, cal2D.c).
+__device__ __forceinline__ int moore_di(int k)
+{
+	return k == 1 || k == 5 || k == 8 ? -1 : (k == 4 || k == 6 || k == 7 ? 1 : 0);
+}
+__device__ __forceinline__ int moore_dj(int k)
+{
+	return k == 2 || k == 5 || k == 6 ? -1 : (k == 3 || k == 7 || k == 8 ? 1 : 0);
+}
+
+// linear index of neighbour k of (i, j), -1 outside the grid
+__device__ __forceinline__ int nb_flat(int i, int j, int k, int rows, int cols)
+{
+	const int ii = i + moore_di(k), jj = j + moore_dj(k);
+	return (ii < 0 || ii >= rows || jj < 0 || jj >= cols) ? -1 : ii * cols + jj;
+}
+
+// grid-stride total: the listed cells, or the whole grid when the list is
+// empty (OpenCAL's A.cells == NULL) or the model has no active cells
+__device__ __forceinline__ size_t sweep_total(int cnt, size_t n) { return cnt > 0 ? (size_t)cnt : n; }
+
+// emission at vent cells; with `flags`, only at cells of the active set
+// (flags equal the list at the start of a transition; an empty list means
+// the whole grid)
+__global__ void sc_vents(double* __restrict__ h, double* __restrict__ T, const uint8_t* __restrict__ flags,
+                         const int* __restrict__ count, VentArgs v, double PTvent)
+{
+	const int k = threadIdx.x;
+	if (k >= v.n) return;
+	const int c = v.cell[k];
+	if (flags && count[0] > 0 && !flags[c]) return;
+	h[c] = h[c] + v.e[k];
+	T[c] = PTvent;
+}
+
+// the same, membership from the list itself (used when the flags were changed
+// by the host after the last rebuild)
+__global__ void sc_vents_list(double* __restrict__ h, double* __restrict__ T, const int* __restrict__ cells,
+                              const int* __restrict__ count, VentArgs v, double PTvent, size_t n)
+{
+	const int cnt = count[0];
+	const size_t total = sweep_total(cnt, n);
+	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+	     idx += (size_t)gridDim.x * blockDim.x) {
+		const int c = cnt > 0 ? cells[idx] : (int)idx;
+		for (int k = 0; k < v.n; ++k)
+			if (c == v.cell[k]) {
+				h[c] = h[c] + v.e[k];
+				T[c] = PTvent;
+			}
+	}
+}
+
+// set the flag of cell c; true when it was clear
+__device__ __forceinline__ bool set_flag(uint8_t* flags, int c)
+{
+	unsigned* w = reinterpret_cast<unsigned*>(flags + (c & ~3));
+	const unsigned bit = 1u << ((c & 3) * 8);
+	return !(atomicOr(w, bit) & bit);
+}
+
+// outflowsMin + empiricalFlows (Sciara.cpp:23-105)
+template <bool ACTIVE>
+__global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ z, const double* __restrict__ h,
+                                                     const double* __restrict__ T, Flows8 f, uint8_t* flags,
+                                                     int* cells, int* count, int rows, int cols, SciaraParams P,
+                                                     int append)
+{
+	const size_t n = (size_t)rows * cols;
+	const int cnt0 = ACTIVE ? count[0] : 0;
+	const size_t total = sweep_total(cnt0, n);
+	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+	     idx += (size_t)gridDim.x * blockDim.x) {
+		const int c = cnt0 > 0 ? cells[idx] : (int)idx;
+		const double h0 = h[c];
+		if (!(h0 > 0)) continue;
+		const int i = c / cols, j = c - (c / cols) * cols;
+		const double t = T[c];
+		const double Pr = pm_pow10(P.a + P.b * t);
+		const double hc = pm_pow10(P.c + P.d * t);
+		const double sz0 = z[c];
+		int nb[9];
+		double zz[9], hh[9], H[9], th[9];
+#pragma unroll
+		for (int k = 0; k < 9; ++k) {
+			nb[k] = k == 0 ? c : nb_flat(i, j, k, rows, cols);
+			hh[k] = nb[k] >= 0 ? h[nb[k]] : 0.0;
+			const double zk = nb[k] >= 0 ? z[nb[k]] : 0.0;
+			zz[k] = k < 5 ? zk : sz0 - (sz0 - zk) / P.rad2;
+			H[k] = 0.0;
+			th[k] = 0.0;
+		}
+		H[0] = zz[0];
+		unsigned keep = 1u;
+		const double lvl0 = zz[0] + hh[0];
+#pragma unroll
+		for (int k = 1; k < 9; ++k)
+			if (lvl0 > zz[k] + hh[k]) {
+				H[k] = zz[k] + hh[k];
+				th[k] = pm_atan((lvl0 - (zz[k] + hh[k])) / P.Pc);
+				keep |= 1u << k;
+			}
+		double avg;
+		bool loop;
+		do {
+			loop = false;
+			avg = hh[0];
+			int counter = 0;
+#pragma unroll
+			for (int k = 0; k < 9; ++k)
+				if ((keep >> k) & 1u) {
+					avg += H[k];
+					counter++;
+				}
+			if (counter != 0) avg = avg / (double)counter;
+#pragma unroll
+			for (int k = 0; k < 9; ++k)
+				if (((keep >> k) & 1u) && avg <= H[k]) {
+					keep &= ~(1u << k);
+					loop = true;
+				}
+		} while (loop);
+		const int base = cnt0;
+#pragma unroll
+		for (int k = 1; k < 9; ++k) {
+			if (!((keep >> k) & 1u)) continue;
+			if (!(h0 > hc * pm_cos(th[k]))) continue;
+			const double fk = Pr * (avg - H[k]);
+			if (fk > 0) {
+				f.f[k - 1][c] = fk;
+				if (ACTIVE && nb[k] >= 0 && set_flag(flags, nb[k]) && append)
+					cells[base + atomicAdd(&count[1], 1)] = nb[k];
+			}
+		}
+	}
+}
+
+// width_update + updateTemperature + removeActiveCells (Sciara.cpp:106-175)
+template <bool ACTIVE>
+__global__ __launch_bounds__(SC_BLOCK) void sc_width(double* __restrict__ z, double* __restrict__ h,
+                                                     const double* __restrict__ T, double* __restrict__ Tn,
+                                                     Flows8 f, const uint8_t* __restrict__ Mb,
+                                                     double* __restrict__ Msl, uint8_t* __restrict__ flags,
+                                                     const int* __restrict__ cells, const int* __restrict__ count,
+                                                     int rows, int cols, SciaraParams P)
+{
+	// the flow that enters from neighbour n is that neighbour's outflow
+	// towards this cell (Sciara.cpp:107)
+	constexpr int opposite[NF] = {3, 2, 1, 0, 6, 7, 4, 5};
+	const size_t n = (size_t)rows * cols;
+	const int cnt = ACTIVE ? count[0] + count[1] : 0;
+	const size_t total = sweep_total(cnt, n);
+	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+	     idx += (size_t)gridDim.x * blockDim.x) {
+		const int c = cnt > 0 ? cells[idx] : (int)idx;
+		const int i = c / cols, j = c - (c / cols) * cols;
+		const double ih = h[c], it = T[c];
+		double ht = 0.0, in_sum = 0.0, out_sum = 0.0;
+#pragma unroll
+		for (int k = 1; k < 9; ++k) {
+			const int q = nb_flat(i, j, k, rows, cols);
+			const double in_flow = q >= 0 ? f.f[opposite[k - 1]][q] : 0.0;
+			const double out_flow = f.f[k - 1][c];
+			const double nt = q >= 0 ? T[q] : 0.0;
+			ht += in_flow * nt;
+			in_sum += in_flow;
+			out_sum += out_flow;
+		}
+		double h1 = ih;
+		h1 += in_sum - out_sum;
+		double t1 = it;
+		if (in_sum > 0 || out_sum > 0) {
+			double residual = ih;
+			residual -= out_sum;
+			t1 = (residual * it + ht) / (residual + in_sum);
+		}
+		double h2 = h1, t2 = t1;
+		const bool mb = Mb[c] != 0;
+		if (h1 > 0 && !mb) {
+			const double nT = t1;
+			const double num = (((3.0 * (nT * nT * nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
+			const double den = ((P.Prho * P.Pcv) * h1) * P.Pac;
+			const double aus = 1.0 + num / den;
+			const double st = nT / pm_cbrt_pos(aus);
+			t2 = st;
+			if (st <= P.PTsol) {
+				z[c] = z[c] + h1;
+				Msl[c] = Msl[c] + h1;
+				h2 = 0.0;
+				t2 = P.PTsol;
+			}
+		}
+		h[c] = h2;
+		Tn[c] = t2;
+		if (ACTIVE && t2 <= P.PTsol && !mb) flags[c] = 0;
+	}
+}
+
+// End of a transition: T := Tn over the processed cells, survivors appended
+// to `alt`; with STEER also the steering (flows := 0 at the active cells,
+// h = T = 0 on the border mask).
+template <bool ACTIVE, bool STEER>
+__global__ __launch_bounds__(SC_BLOCK) void sc_finish(double* __restrict__ h, double* __restrict__ T,
+                                                      const double* __restrict__ Tn, Flows8 f,
+                                                      const uint8_t* __restrict__ Mb,
+                                                      const uint8_t* __restrict__ flags,
+                                                      const int* __restrict__ cells, int* __restrict__ alt,
+                                                      int* count, const int* __restrict__ mb_cells, int n_mb,
+                                                      int rows, int cols)
+{
+	const size_t n = (size_t)rows * cols;
+	const int cnt = ACTIVE ? count[0] + count[1] : 0;
+	const size_t total = sweep_total(cnt, n);
+	const size_t stride = (size_t)gridDim.x * blockDim.x;
+	const int lane = threadIdx.x & (dev::WAVE - 1);
+	for (size_t base = (size_t)blockIdx.x * blockDim.x; base < total; base += stride) {
+		const size_t idx = base + threadIdx.x;
+		const bool valid = idx < total;
+		const int c = valid ? (cnt > 0 ? cells[idx] : (int)idx) : 0;
+		bool surv = false;
+		if (valid) {
+			const bool mb = Mb[c] != 0;
+			T[c] = (STEER && mb) ? 0.0 : Tn[c];
+			if (ACTIVE) surv = flags[c] != 0;
+			if (STEER && (!ACTIVE || surv))
+#pragma unroll
+				for (int q = 0; q < NF; ++q) f.f[q][c] = 0.0;
+			if (STEER && mb) h[c] = 0.0;
+		}
+		if (ACTIVE) {
+			const unsigned long long m = __ballot(surv);
+			if (m) {
+				const int leader = __ffsll((long long)m) - 1;
+				int pos = 0;
+				if (lane == leader) pos = atomicAdd(&count[2], __popcll(m));
+				pos = __shfl(pos, leader);
+				if (surv) alt[pos + __popcll(m & ((1ull << lane) - 1ull))] = c;
+			}
+		}
+	}
+	if (!ACTIVE) return;
+	if (STEER)
+		for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n_mb; k += stride) {
+			const int c = mb_cells[k];
+			h[c] = 0.0;
+			T[c] = 0.0;
+		}
+	__shared__ int last_block;
+	__shared__ int survivors;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		__threadfence();
+		const int done = atomicAdd(&count[3], 1);
+		last_block = done == (int)gridDim.x - 1;
+		if (last_block) {
+			__threadfence();
+			survivors = atomicAdd(&count[2], 0);
+			atomicExch(&count[0], survivors);
+			atomicExch(&count[1], 0);
+			atomicExch(&count[2], 0);
+			atomicExch(&count[3], 0);
+		}
+	}
+	__syncthreads();
+	// an empty set: the steering's reset covers the whole grid
+	if (STEER && last_block && survivors == 0)
+		for (size_t c = threadIdx.x; c < n; c += blockDim.x)
+#pragma unroll
+			for (int q = 0; q < NF; ++q) f.f[q][c] = 0.0;
+}
+
+// steering alone (the run loop called the twin separately from the transition)
+template <bool ACTIVE>
+__global__ __launch_bounds__(SC_BLOCK) void sc_steer(double* __restrict__ h, double* __restrict__ T, Flows8 f,
+                                                     const uint8_t* __restrict__ Mb, const int* __restrict__ cells,
+                                                     const int* __restrict__ count,
+                                                     const int* __restrict__ mb_cells, int n_mb, int rows,
+                                                     int cols)
+{
+	const size_t n = (size_t)rows * cols;
+	const int cnt = ACTIVE ? count[0] : 0;
+	const size_t total = sweep_total(cnt, n);
+	const size_t stride = (size_t)gridDim.x * blockDim.x;
+	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+		const int c = cnt > 0 ? cells[idx] : (int)idx;
+#pragma unroll
+		for (int q = 0; q < NF; ++q) f.f[q][c] = 0.0;
+		if (!ACTIVE && Mb[c]) {
+			h[c] = 0.0;
+			T[c] = 0.0;
+		}
+	}
+	if (ACTIVE)
+		for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n_mb; k += stride) {
+			const int c = mb_cells[k];
+			h[c] = 0.0;
+			T[c] = 0.0;
+		}
+}
+
+int clamp_grid(size_t work, int lo, int hi)
+{
+	size_t b = (work + SC_BLOCK - 1) / SC_BLOCK;
+	return (int)std::max<size_t>((size_t)lo, std::min<size_t>((size_t)hi, b));
+}
+
+struct SciaraProgram : Program {
+	SubX *z = nullptr, *h = nullptr, *T = nullptr, *mb = nullptr, *msl = nullptr;
+	SubX* f[NF] = {};
+	calhipSciaraBinding b{};
+	std::vector<int> vent_rc;
+	std::vector<double> rates;
+	bool active = false;
+
+	int* alt = nullptr;          // second list buffer (compaction target)
+	int* d_mb = nullptr;         // border-mask cells (linear indices)
+	int n_mb = 0, cap_mb = 0;
+	bool unordered = false;      // the device list is not in row-major order
+	// launch sizing: an estimate of the list length, refreshed from an
+	// asynchronous read-back of count[0] every few transitions
+	int* h_cnt = nullptr;
+	hipEvent_t ev_cnt = nullptr;
+	bool cnt_pending = false;
+	long long est = 0;
+	long long steps_done = 0;
+
+	~SciaraProgram() override
+	{
+		if (alt) (void)hipFree(alt);
+		if (d_mb) (void)hipFree(d_mb);
+		if (h_cnt) (void)hipHostFree(h_cnt);
+		if (ev_cnt) (void)hipEventDestroy(ev_cnt);
+	}
+
+	const char* name() const override { return "SCIARA-fv2"; }
+
+	SciaraParams params() const
+	{
+		SciaraParams P;
+		P.Pc = *b.Pc;
+		P.Pac = *b.Pac;
+		P.a = *b.a;
+		P.b = *b.b;
+		P.c = *b.c;
+		P.d = *b.d;
+		P.rad2 = *b.rad2;
+		P.PTsol = *b.PTsol;
+		P.PTvent = *b.PTvent;
+		P.Pclock = *b.Pclock;
+		P.Pcool = *b.Pcool;
+		P.Prho = *b.Prho;
+		P.Peps = *b.Pepsilon;
+		P.Psigma = *b.Psigma;
+		P.Pcv = *b.Pcv;
+		return P;
+	}
+
+	Flows8 flows() const
+	{
+		Flows8 F;
+		for (int q = 0; q < NF; ++q) F.f[q] = (double*)f[q]->d_cur;
+		return F;
+	}
+
+	int after_upload(ModelX* m) override
+	{
+		size_t n = m->cells();
+		const CALbyte* hm = mb->pub.b.current;
+		std::vector<int> lst;
+		for (size_t c = 0; c < n; ++c)
+			if (hm[c]) lst.push_back((int)c);
+		n_mb = (int)lst.size();
+		if (n_mb > cap_mb) {
+			if (d_mb) (void)hipFree(d_mb);
+			d_mb = nullptr;
+			cap_mb = 0;
+			if (CALHIP_CHECK(hipMalloc(&d_mb, sizeof(int) * (size_t)n_mb))) return -1;
+			cap_mb = n_mb;
+		}
+		if (n_mb && CALHIP_CHECK(hipMemcpyAsync(d_mb, lst.data(), sizeof(int) * (size_t)n_mb,
+		                                        hipMemcpyHostToDevice, m->stream)))
+			return -1;
+		if (active) {
+			if (!alt && CALHIP_CHECK(hipMalloc(&alt, sizeof(int) * n))) return -1;
+			if (!h_cnt && CALHIP_CHECK(hipHostMalloc((void**)&h_cnt, sizeof(int), hipHostMallocDefault)))
+				return -1;
+			if (!ev_cnt && CALHIP_CHECK(hipEventCreateWithFlags(&ev_cnt, hipEventDisableTiming))) return -1;
+			// counters 1..3 start cleared; count[0] came with the list
+			if (CALHIP_CHECK(hipMemsetAsync(m->d_count + 1, 0, 3 * sizeof(int), m->stream))) return -1;
+			est = m->pub.A.cells ? m->pub.A.size_current : 0;
+			cnt_pending = false;
+		}
+		unordered = false;
+		return CALHIP_CHECK(hipStreamSynchronize(m->stream));
+	}
+
+	int sync_active(ModelX* m) override
+	{
+		if (!unordered) return 0;
+		// flags equal the set; rebuild the row-major list from them
+		if (k::rebuild_active(m)) return -1;
+		unordered = false;
+		return 0;
+	}
+
+	int check_entry(ModelX* m)
+	{
+		SubX* dl[3 + NF] = {z, h, T};
+		for (int q = 0; q < NF; ++q) dl[3 + q] = f[q];
+		for (SubX* s : dl)
+			if (s->nxt_dirty || s->tstate != Transient::None) {
+				set_error(10, "SCIARA device program: a substate has next-layer writes pending at the start of a "
+				              "step (call calUpdate2D first)");
+				return -1;
+			}
+		(void)m;
+		return 0;
+	}
+
+	// emission of every vent for the current elapsed time (vent.h:74-82),
+	// one entry per distinct cell: the last listed vent with a positive
+	// thickness sets it (Sciara.cpp:5-15 overwrite the next layer)
+	void vent_list(std::vector<std::pair<int, double>>& out, int cols) const
+	{
+		out.clear();
+		const double el = *b.elapsed_time;
+		const unsigned et = *b.emission_time;
+		const unsigned i = (unsigned)(el / et);
+		for (int v = 0; v < b.n_vents; ++v) {
+			double e = i >= (unsigned)b.n_rates ? 0.0 : rates[(size_t)v * b.n_rates + i] / *b.Pac * *b.Pclock;
+			if (!(e > 0)) continue;
+			int c = vent_rc[2 * v] * cols + vent_rc[2 * v + 1];
+			bool found = false;
+			for (auto& p : out)
+				if (p.first == c) {
+					p.second = e;
+					found = true;
+				}
+			if (!found) out.push_back({c, e});
+		}
+	}
+
+	int launch_vents(ModelX* m, bool by_list)
+	{
+		std::vector<std::pair<int, double>> vl;
+		vent_list(vl, m->pub.columns);
+		for (size_t s = 0; s < vl.size(); s += VENTS_PER_LAUNCH) {
+			VentArgs va;
+			va.n = (int)std::min<size_t>(VENTS_PER_LAUNCH, vl.size() - s);
+			for (int k = 0; k < va.n; ++k) {
+				va.cell[k] = vl[s + k].first;
+				va.e[k] = vl[s + k].second;
+			}
+			if (by_list) {
+				hipLaunchKernelGGL(sc_vents_list, dim3(clamp_grid((size_t)est + 1, 1, 1024)), dim3(SC_BLOCK), 0,
+				                   m->stream, (double*)h->d_cur, (double*)T->d_cur, m->d_cells, m->d_count, va,
+				                   *b.PTvent, m->cells());
+			} else {
+				hipLaunchKernelGGL(sc_vents, dim3(1), dim3(64), 0, m->stream, (double*)h->d_cur,
+				                   (double*)T->d_cur, active ? (const uint8_t*)m->d_flags : nullptr,
+				                   active ? (const int*)m->d_count : nullptr, va, *b.PTvent);
+			}
+			m->stats.launches++;
+		}
+		return 0;
+	}
+
+	void refresh_estimate()
+	{
+		if (cnt_pending && hipEventQuery(ev_cnt) == hipSuccess) {
+			est = *h_cnt;
+			cnt_pending = false;
+		}
+	}
+
+	int list_grid(ModelX* m) const
+	{
+		if (!active) return clamp_grid(m->cells(), 1, 8192);
+		// room for growth between read-backs; the kernels loop over any excess
+		return clamp_grid((size_t)(2 * est + 4096), 16, 8192);
+	}
+
+	int transition(ModelX* m, bool steer)
+	{
+		if (check_entry(m)) return -1;
+		refresh_estimate();
+		const SciaraParams P = params();
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		const Flows8 F = flows();
+		hipEvent_t ev;
+		timer_begin(m, &ev);
+		// host changes to the set since the last rebuild: the vent process
+		// still sees the old list, then the rebuild applies them
+		const bool pending_host = active && m->flags_dirty;
+		if (launch_vents(m, pending_host)) return -1;
+		if (pending_host) {
+			if (device_rebuild_active(m)) return -1;
+			unordered = false;
+		}
+		const int grid = list_grid(m);
+		if (active)
+			hipLaunchKernelGGL(sc_flows<true>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (const double*)z->d_cur,
+			                   (const double*)h->d_cur, (const double*)T->d_cur, F, m->d_flags, m->d_cells,
+			                   m->d_count, rows, cols, P, 1);
+		else
+			hipLaunchKernelGGL(sc_flows<false>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (const double*)z->d_cur,
+			                   (const double*)h->d_cur, (const double*)T->d_cur, F, nullptr, nullptr, nullptr,
+			                   rows, cols, P, 0);
+		double* Tn = (double*)T->d_nxt;
+		if (active)
+			hipLaunchKernelGGL(sc_width<true>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (double*)z->d_cur,
+			                   (double*)h->d_cur, (const double*)T->d_cur, Tn, F, (const uint8_t*)mb->d_cur,
+			                   (double*)msl->d_cur, m->d_flags, m->d_cells, m->d_count, rows, cols, P);
+		else
+			hipLaunchKernelGGL(sc_width<false>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (double*)z->d_cur,
+			                   (double*)h->d_cur, (const double*)T->d_cur, Tn, F, (const uint8_t*)mb->d_cur,
+			                   (double*)msl->d_cur, nullptr, nullptr, nullptr, rows, cols, P);
+		if (active) {
+			if (steer)
+				hipLaunchKernelGGL((sc_finish<true, true>), dim3(grid), dim3(SC_BLOCK), 0, m->stream,
+				                   (double*)h->d_cur, (double*)T->d_cur, (const double*)Tn, F,
+				                   (const uint8_t*)mb->d_cur, (const uint8_t*)m->d_flags, m->d_cells, alt,
+				                   m->d_count, d_mb, n_mb, rows, cols);
+			else
+				hipLaunchKernelGGL((sc_finish<true, false>), dim3(grid), dim3(SC_BLOCK), 0, m->stream,
+				                   (double*)h->d_cur, (double*)T->d_cur, (const double*)Tn, F,
+				                   (const uint8_t*)mb->d_cur, (const uint8_t*)m->d_flags, m->d_cells, alt,
+				                   m->d_count, d_mb, n_mb, rows, cols);
+			std::swap(m->d_cells, alt);
+			unordered = true;
+		} else {
+			if (steer)
+				hipLaunchKernelGGL((sc_finish<false, true>), dim3(grid), dim3(SC_BLOCK), 0, m->stream,
+				                   (double*)h->d_cur, (double*)T->d_cur, (const double*)Tn, F,
+				                   (const uint8_t*)mb->d_cur, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+				                   rows, cols);
+			else
+				hipLaunchKernelGGL((sc_finish<false, false>), dim3(grid), dim3(SC_BLOCK), 0, m->stream,
+				                   (double*)h->d_cur, (double*)T->d_cur, (const double*)Tn, F,
+				                   (const uint8_t*)mb->d_cur, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+				                   rows, cols);
+		}
+		m->stats.launches += 3;
+		if (steer) *b.elapsed_time += *b.Pclock;
+		timer_end(m, ev);
+		if (CALHIP_CHECK(hipGetLastError())) return -1;
+		if (active && (++steps_done & 15) == 0 && !cnt_pending) {
+			if (CALHIP_CHECK(hipMemcpyAsync(h_cnt, m->d_count, sizeof(int), hipMemcpyDeviceToHost, m->stream)) ||
+			    CALHIP_CHECK(hipEventRecord(ev_cnt, m->stream)))
+				return -1;
+			cnt_pending = true;
+		}
+		// every layer was written in place; the next layers are logically
+		// equal to the current ones (St's next buffer held scratch)
+		T->nxt_stale = true;
+		return 0;
+	}
+
+	int step(ModelX* m) override { return transition(m, false); }
+	int step_and_steer(ModelX* m) override { return transition(m, true); }
+
+	// Sciara.cpp:185-197 on the device
+	int steer(ModelX* m) override
+	{
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		refresh_estimate();
+		const int grid = list_grid(m);
+		if (active && m->flags_dirty) {
+			// the reset covers the list as OpenCAL holds it (rebuilt at updates)
+			set_error(11, "SCIARA steering twin: the active set has changes pending (call calUpdate2D first)");
+			return -1;
+		}
+		if (active)
+			hipLaunchKernelGGL(sc_steer<true>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (double*)h->d_cur,
+			                   (double*)T->d_cur, flows(), (const uint8_t*)mb->d_cur, m->d_cells, m->d_count, d_mb,
+			                   n_mb, rows, cols);
+		else
+			hipLaunchKernelGGL(sc_steer<false>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (double*)h->d_cur,
+			                   (double*)T->d_cur, flows(), (const uint8_t*)mb->d_cur, nullptr, nullptr, nullptr, 0,
+			                   rows, cols);
+		m->stats.launches++;
+		*b.elapsed_time += *b.Pclock;
+		return CALHIP_CHECK(hipGetLastError());
+	}
+};
+
+bool real_layered(struct CALSubstate2Dr* q) { return q && S(q)->type == 'r' && !S(q)->single; }
+
+}  // namespace
+
+}  // namespace calhip
+
+using namespace calhip;
+
+extern "C" int calhipBindSciara2D(struct CALModel2D* ca2D, const struct calhipSciaraBinding* b)
+{
+	if (!ca2D || !b) {
+		set_error(30, "calhipBindSciara2D: missing argument");
+		return -1;
+	}
+	ModelX* m = X(ca2D);
+	if (ca2D->X_id != CAL_MOORE_NEIGHBORHOOD_2D || ca2D->T != CAL_SPACE_FLAT) {
+		set_error(31, "calhipBindSciara2D: the SCIARA program needs a Moore, CAL_SPACE_FLAT model");
+		return -1;
+	}
+	bool ok = real_layered(b->Sz) && real_layered(b->Slt) && real_layered(b->St) && b->Mb &&
+	          S(b->Mb)->type == 'b' && b->Msl && S(b->Msl)->type == 'r';
+	for (int q = 0; q < NF; ++q) ok = ok && real_layered(b->f[q]);
+	const CALreal* const prm[] = {b->Pclock, b->PTsol, b->PTvent, b->Pc, b->Pac, b->a, b->b, b->c,
+	                              b->d, b->rad2, b->Pcool, b->Prho, b->Pepsilon, b->Psigma, b->Pcv};
+	for (const CALreal* p : prm) ok = ok && p;
+	ok = ok && b->vents && b->flows && b->width && b->temperature && b->elapsed_time && b->emission_time;
+	ok = ok && b->n_vents >= 0 && b->n_rates >= 0 && (b->n_vents == 0 || b->vent_rc) &&
+	     (b->n_vents * b->n_rates == 0 || b->rates);
+	if (!ok) {
+		set_error(32, "calhipBindSciara2D: missing or mistyped substate, process or parameter (Sz, Slt, St, f[8] "
+		              "double-layer real; Mb byte; Msl real)");
+		return -1;
+	}
+	const bool active = ca2D->OPTIMIZATION == CAL_OPT_ACTIVE_CELLS;
+	if (active != (b->remove != nullptr)) {
+		set_error(33, "calhipBindSciara2D: the removal process is required with active cells and only then");
+		return -1;
+	}
+	for (int v = 0; v < b->n_vents; ++v)
+		if (b->vent_rc[2 * v] < 0 || b->vent_rc[2 * v] >= ca2D->rows || b->vent_rc[2 * v + 1] < 0 ||
+		    b->vent_rc[2 * v + 1] >= ca2D->columns) {
+			set_error(34, "calhipBindSciara2D: a vent lies outside the grid");
+			return -1;
+		}
+	auto* p = new SciaraProgram();
+	p->b = *b;
+	p->z = S(b->Sz);
+	p->h = S(b->Slt);
+	p->T = S(b->St);
+	for (int q = 0; q < NF; ++q) p->f[q] = S(b->f[q]);
+	p->mb = S(b->Mb);
+	p->msl = S(b->Msl);
+	p->active = active;
+	p->vent_rc.assign(b->vent_rc, b->vent_rc + 2 * (size_t)b->n_vents);
+	p->rates.assign(b->rates, b->rates + (size_t)b->n_vents * b->n_rates);
+	p->b.vent_rc = p->vent_rc.data();
+	p->b.rates = p->rates.data();
+	delete m->prog;
+	m->prog = p;
+	m->prog_eps.clear();
+	m->prog_eps.push_back(b->vents);
+	m->prog_eps.push_back(b->flows);
+	m->prog_eps.push_back(b->width);
+	m->prog_eps.push_back(b->temperature);
+	if (b->remove) m->prog_eps.push_back(b->remove);
+	m->prog_steering = b->steering;
+	// a model already on the device gets the program's set-up now
+	if (m->phase == Phase::Device && p->after_upload(m)) return -1;
+	return 0;
+}

@@ -530,6 +530,7 @@ extern "C" int calhipBindSciddicaT2D(struct CALModel2D* ca2D, CALCallbackFunc2D 
 	delete m->prog;
 	m->prog = p;
 	m->prog_eps.clear();
+	m->prog_steering = nullptr;
 	m->prog_eps.push_back(flows);
 	m->prog_eps.push_back(width);
 	if (remove) m->prog_eps.push_back(remove);

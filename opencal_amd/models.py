@@ -177,3 +177,82 @@ def sciddicat_batch(instances, steps):
     rc = L.calhipSciddicaTBatchRun(n, rows, cols, pr, pe, zp, hp, int(steps), op, C.byref(ms))
     check(rc, "calhipSciddicaTBatchRun")
     return outs, ms.value
+
+
+class Sciara:
+    """One SCIARA-fv2 model (opencal_amd/examples/sciara.c) on the GPU.
+
+    inp: the dict of synth.sciara_inputs (z0, cell_size, params, vents, rates,
+    emission_time).  twin=False leaves the steering to the model's host
+    callback (every step then reads the border mask on the host)."""
+
+    def __init__(self, inp, steps, active=True, twin=True):
+        self.z0 = np.ascontiguousarray(inp["z0"], np.float64)
+        self.rows, self.cols = self.z0.shape
+        self.params = np.ascontiguousarray(inp["params"], np.float64)
+        self.vents = np.ascontiguousarray(inp["vents"], np.int32)
+        self.rates = np.ascontiguousarray(inp["rates"], np.float64)
+        self.E = examples()
+        lib().calhipClearError()
+        self.handle = self.E.sciara_create(self.rows, self.cols, _d(self.z0), float(inp["cell_size"]),
+                                           _d(self.params), int(self.vents.shape[0]),
+                                           self.vents.ctypes.data_as(_ip), int(self.rates.shape[1]),
+                                           _d(self.rates), int(inp["emission_time"]), int(steps),
+                                           int(bool(active)), 1 if twin else 2)
+        if not self.handle:
+            check(-1, "sciara_create")
+        self.steps_run = None
+
+    @property
+    def model(self):
+        return self.E.sciara_model(self.handle)
+
+    def run(self):
+        r = self.E.sciara_run(self.handle)
+        check(-1 if r < 0 else 0, "sciara_run")
+        self.steps_run = r
+        return self
+
+    def read(self):
+        out = {}
+        for k, name in enumerate(("z", "h", "T", "Msl")):
+            a = np.empty((self.rows, self.cols), np.float64)
+            check(self.E.sciara_read(self.handle, k, _d(a)), "sciara_read")
+            out[name] = a
+        out["steps"] = self.steps_run
+        out["active"] = self.E.sciara_active_count(self.handle)
+        out["elapsed"] = self.E.sciara_elapsed(self.handle)
+        return out
+
+    def stats(self):
+        return stats(self.model)
+
+    def timing(self, enable):
+        lib().calhipEnableKernelTiming2D(self.model, int(enable))
+
+    def kernel_time(self):
+        ms = C.c_double()
+        n = C.c_longlong()
+        lib().calhipGetKernelTime2D(self.model, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def close(self):
+        if self.handle:
+            self.E.sciara_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sciara_run(inp, steps, active=True, twin=True):
+    """calRun2D of a SCIARA-fv2 model; returns the dict of Sciara.read()."""
+    m = Sciara(inp, steps, active, twin)
+    try:
+        m.run()
+        return m.read()
+    finally:
+        m.close()

@@ -130,7 +130,7 @@ static void ep_flows(sciara_t* s, int i, int j)
 	for (k = 1; k < 9; k++)
 		if (f[k] > 0) {
 			s->q[F0 + k - 1].nxt[c] = f[k];
-			if (g->active_mode) orc_add_active(&s->g, nbs[k]);
+			if (g->active_mode && nbs[k] >= 0) orc_add_active(&s->g, nbs[k]);
 		}
 }
 

@@ -246,3 +246,35 @@ def test_unbound_transition_is_an_error(libs):
     assert mine.calGet2Dr(m, q, 3, 3) == 2.0
     L.calhipClearError()
     mine.calFinalize2D(m)
+
+
+def test_sciara_binding_validates_and_fails_loudly_without_gpu():
+    """The SCIARA example binds on the host (the binding's checks pass for a
+    model built as the reference builds it); wrong model shapes are refused;
+    with no GPU visible the run reports an error instead of computing."""
+    import torch
+    from opencal_amd import lib
+    from opencal_amd.models import Sciara
+    from opencal_amd import synth
+    inp = synth.sciara_inputs(32, 40, 1, steps_per_rate=5, n_rates=2)
+    m = Sciara(inp, 10, active=True)   # binds; no device call yet
+    L = lib()
+    assert L.calhipGetLastError() == 0
+    if not torch.cuda.is_available():
+        assert m.E.sciara_run(m.handle) == -1
+        assert L.calhipGetLastError() != 0   # no device: a HIP error, nothing computed
+    m.close()
+    L.calhipClearError()
+    # a von Neumann model is refused
+    P = C.c_void_p
+    L.calCADef2D.restype = P
+    vn = L.calCADef2D(8, 8, 1, 0, 0)
+    class Binding(C.Structure):
+        _fields_ = [("ptrs", C.c_void_p * 64)]
+    b = Binding()
+    L.calhipBindSciara2D.argtypes = [P, C.POINTER(Binding)]
+    assert L.calhipBindSciara2D(vn, C.byref(b)) == -1
+    assert L.calhipGetLastError() == 31
+    L.calhipClearError()
+    L.calFinalize2D.argtypes = [P]
+    L.calFinalize2D(vn)

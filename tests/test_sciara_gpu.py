@@ -1,0 +1,103 @@
+"""SCIARA-fv2 on the GPU through the OpenCAL C API (examples/sciara.c bound
+with calhipBindSciara2D).
+
+Pins: (1) bit-exact agreement with the oracle run in portable-math mode (the
+same elementary functions, host-built) on seeded cases, active cells and
+full grid, including two vents on one cell and runs ended by the stop
+condition; (2) the steering twin equals the model's host steering bit for
+bit; (3) the libm oracle (bit-exact to the reference build on the CPU) within
+1e-9 relative: the elementary functions differ from glibc by a few ulp;
+(4) at 1024 x 1024 (size-independent properties): lava volume balance to
+1e-12 relative while lava stays off the border, temperature within
+[PTsol, PTvent], solidified altitude = initial altitude + Msl."""
+import numpy as np
+import pytest
+
+import native
+from opencal_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(rows, cols, seed, vents, steps_per_rate=60, n_rates=4):
+    inp = synth.sciara_inputs(rows, cols, seed, steps_per_rate=steps_per_rate, n_rates=n_rates)
+    inp["vents"] = np.ascontiguousarray(np.array(vents, dtype=np.int32))
+    inp["rates"] = np.ascontiguousarray(inp["rates"][: len(vents)])
+    return inp
+
+
+CASES = [
+    ("3 vents 64x96", 64, 96, 7, [[14, 40], [15, 22], [20, 60]], 40, 5, 100000),
+    ("1 vent 80x72", 80, 72, 3, [[20, 36]], 50, 4, 100000),
+    ("shared vent 96x128", 96, 128, 11, [[30, 50], [30, 50], [44, 90]], 60, 4, 100000),
+    ("bounded steps 128x160", 128, 160, 5, [[40, 80], [50, 70]], 200, 3, 333),
+]
+
+
+def _same(dev, ora, label):
+    assert dev["steps"] == ora["steps"], label
+    for k in ("z", "h", "T", "Msl"):
+        assert np.array_equal(dev[k], ora[k]), f"{label}: {k} max diff {np.abs(dev[k] - ora[k]).max()}"
+
+
+@pytest.mark.parametrize("active", [True, False], ids=["active", "full"])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_sciara_bit_exact_vs_portable_oracle(case, active):
+    from opencal_amd.models import sciara_run
+    label, rows, cols, seed, vents, spr, nr, steps = case
+    print(f"case sciara {label} active={active}", flush=True)
+    inp = _inputs(rows, cols, seed, vents, spr, nr)
+    dev = sciara_run(inp, steps, active)
+    ora = native.oracle_sciara(inp, steps, active, portable=True)
+    _same(dev, ora, label)
+    if active:
+        assert dev["active"] == ora["active"]
+    assert (dev["Msl"] > 0).any()
+
+
+def test_sciara_twin_equals_host_steering():
+    from opencal_amd.models import sciara_run
+    print("case sciara twin vs host steering", flush=True)
+    inp = _inputs(64, 80, 17, [[20, 40], [24, 30]], 30, 3)
+    a = sciara_run(inp, 1000, True, twin=True)
+    b = sciara_run(inp, 1000, True, twin=False)
+    _same(a, b, "twin")
+    assert a["active"] == b["active"] and a["elapsed"] == b["elapsed"] == 90 * inp["params"][0]
+
+
+def test_sciara_close_to_libm_oracle():
+    from opencal_amd.models import sciara_run
+    print("case sciara vs libm oracle", flush=True)
+    inp = _inputs(96, 96, 13, [[30, 48], [36, 40]], 50, 4)
+    dev = sciara_run(inp, 1000, True)
+    ref = native.oracle_sciara(inp, 1000, True, portable=False)
+    assert dev["steps"] == ref["steps"]
+    for k, scale in (("h", 1.0), ("T", 1400.0), ("z", 3000.0), ("Msl", 1.0)):
+        assert np.abs(dev[k] - ref[k]).max() <= 1e-9 * scale, k
+
+
+def test_sciara_bit_exact_256_active():
+    from opencal_amd.models import sciara_run
+    print("case sciara 256x256 1200 steps", flush=True)
+    inp = synth.sciara_inputs(256, 256, 21, steps_per_rate=400, n_rates=3)
+    dev = sciara_run(inp, 100000, True)
+    ora = native.oracle_sciara(inp, 100000, True, portable=True)
+    _same(dev, ora, "256")
+    assert dev["active"] == ora["active"]
+
+
+def test_sciara_properties_1024():
+    from opencal_amd.models import sciara_run
+    print("case sciara 1024x1024 3000 steps properties", flush=True)
+    inp = synth.sciara_inputs(1024, 1024, 2, steps_per_rate=1000, n_rates=3)
+    dev = sciara_run(inp, 100000, True)
+    assert dev["steps"] == 3000
+    h, msl, T = dev["h"], dev["Msl"], dev["T"]
+    lava = np.argwhere(h > 0)
+    assert lava.size and lava.min() > 2 and lava[:, 0].max() < 1021 and lava[:, 1].max() < 1021
+    pac = inp["cell_size"] ** 2
+    em = (inp["rates"] * inp["params"][0] / pac).sum() * 1000
+    assert abs(h.sum() + msl.sum() - em) <= 1e-12 * em
+    assert (T[h > 0] > inp["params"][1]).all() and (T[h > 0] <= inp["params"][2]).all()
+    assert np.allclose(dev["z"] - msl, inp["z0"], rtol=0, atol=1e-9)
+    assert (msl > 0).sum() > 0
