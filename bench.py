@@ -17,7 +17,8 @@ The same JSON line carries
                 1024^2 x 4000, Life 512^2 x 1000, the batched ensemble of 256
                 uneven instances x 2000 steps against the same instances run
                 one after another, and the adjoint gradient / forward time
-                ratio at 1024^2 x 2000 with checkpoints every 45 steps.
+                ratio at 1024^2 x 2000 with checkpoints every 45 steps, and
+                SCIARA-fv2 2048^2 with active cells for 10000 steps.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -222,6 +223,48 @@ def bench_adjoint(n=1024, steps=2000, ck=45, seed=5):
             "fd_dL_dpr": fd, "fd_rel_err": abs(g - fd) / max(abs(fd), 1e-30)}
 
 
+def bench_sciara(n=2048, seed=3, cpu_steps=200):
+    """SCIARA-fv2 lava flow (config: 2048^2, Moore, active cells, 3 vents, 10
+    emission intervals of 1000 steps; the model's stop condition ends the run
+    after 10000 steps), stepped through calRunCAStep2D with the steering twin,
+    model resident on the GPU after the first step."""
+    import native
+    from opencal_amd import synth
+    from opencal_amd.models import Sciara
+    inp = synth.sciara_inputs(n, n, seed)
+    total = int(inp["emission_time"] / inp["params"][0]) * inp["rates"].shape[1]
+    m = Sciara(inp, total + 10, active=True)
+    m.init()
+    m.steps(1)
+    m.sync()
+    t0 = time.perf_counter()
+    ran = m.steps(total + 10)
+    m.sync()
+    wall = time.perf_counter() - t0
+    res = m.read()
+    st = m.stats()
+    m.close()
+    steps = ran + 1
+    out = {"workload": f"sciara_fv2_{n}x{n}_active_x{steps}", "steps": steps, "ms_per_step": wall / ran * 1e3,
+           "final_active_cells": res["active"],
+           "lava_cells": int((res["h"] > 0).sum()), "solidified_cells": int((res["Msl"] > 0).sum()),
+           "launches_per_step": st["launches"] / max(st["steps"], 1)}
+    # lava volume balance (no lava reaches the border mask in this run)
+    pac = inp["cell_size"] ** 2
+    em = float((inp["rates"] * inp["params"][0] / pac).sum() * (inp["emission_time"] / inp["params"][0]))
+    out["volume_balance_rel"] = float((res["h"].sum() + res["Msl"].sum() - em) / em)
+    # CPU: the oracle restatement (libm, as the reference) on the first
+    # cpu_steps steps of the same run; its cost per step is dominated by the
+    # full-grid active-set rebuilds and barely depends on the lava extent
+    nt = native.set_threads(int(os.environ.get("OMP_NUM_THREADS", "16")))
+    t0 = time.perf_counter()
+    native.oracle_sciara(inp, cpu_steps, True, portable=False)
+    cw = time.perf_counter() - t0
+    out["cpu_port_ms_per_step"] = cw / cpu_steps * 1e3
+    out["cpu_sample"] = f"oracle (libm mode) first {cpu_steps} steps incl. init, {nt} threads"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +274,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--only", choices=["active", "life", "batch", "adjoint", "sciara"],
+                    help="run one secondary workload alone and print its result (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on one MI355X; "
@@ -239,6 +284,11 @@ def main():
 
     import torch
     assert torch.cuda.is_available(), "needs a GPU"
+    workloads = {"active": bench_active, "life": bench_life, "batch": bench_batch, "adjoint": bench_adjoint,
+                 "sciara": bench_sciara}
+    if args.only:
+        print(json.dumps({args.only: workloads[args.only]()}), flush=True)
+        return
     log(f"sciddicaT plain {args.size}^2: warmup {args.warmup}, timed {args.steps}")
     r = bench_sciddicat_plain(args.size, args.steps, args.warmup)
     log(json.dumps(r))
@@ -268,8 +318,7 @@ def main():
         out["cpu_baseline"] = cpu_baseline_sciddicat(args.size, args.cpu_steps)
     if not args.quick:
         wl = {}
-        for name, fn in (("active", bench_active), ("life", bench_life), ("batch", bench_batch),
-                         ("adjoint", bench_adjoint)):
+        for name, fn in workloads.items():
             log(f"{name} ...")
             t0 = time.perf_counter()
             try:

@@ -106,6 +106,9 @@ def _declare_examples(E):
     E.sciara_create.argtypes = [C.c_int, C.c_int, dp, C.c_double, dp, C.c_int, ip, C.c_int, dp, C.c_uint,
                                 C.c_int, C.c_int, C.c_int]
     E.sciara_run.argtypes = [vp]
+    E.sciara_init.argtypes = [vp]
+    E.sciara_steps.argtypes = [vp, C.c_int]
+    E.sciara_sync.argtypes = [vp]
     E.sciara_read.argtypes = [vp, C.c_int, dp]
     E.sciara_active_count.argtypes = [vp]
     E.sciara_elapsed.restype = C.c_double

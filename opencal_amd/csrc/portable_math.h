@@ -121,21 +121,20 @@ PM_FN double pm_pow10(double x)
 /* x^(1/3) for x > 0, evaluated as exp(log(x) * (1/3)) */
 PM_FN double pm_cbrt_pos(double x) { return pm_exp(pm_log(x) * (1.0 / 3.0)); }
 
+/* atan and cos below are branch-free (selects only), so that independent
+ * calls interleave on the GPU; each select picks a value computed exactly as
+ * a branch would compute it. */
 PM_FN double pm_atan(double x)
 {
-	double t = x < 0.0 ? -x : x, y, y2, p, res, off = 0.0;
-	int inv = 0;
-	if (t > 1.0) {
-		t = 1.0 / t;
-		inv = 1;
-	}
-	if (t > PM_TAN_PI_12) {
-		y = (t * PM_SQRT3 - 1.0) / (t + PM_SQRT3);
-		off = PM_PI_6;
-	} else {
-		y = t;
-	}
-	y2 = y * y;
+	const double ax = x < 0.0 ? -x : x;
+	const int inv = ax > 1.0;
+	const double t = inv ? 1.0 / ax : ax;
+	const int red = t > PM_TAN_PI_12;
+	const double yr = (t * PM_SQRT3 - 1.0) / (t + PM_SQRT3);
+	const double y = red ? yr : t;
+	const double off = red ? PM_PI_6 : 0.0;
+	const double y2 = y * y;
+	double p, res;
 	/* y - y^3/3 + y^5/5 - ... - y^27/27 */
 	p = -1.0 / 27.0;
 	p = p * y2 + 1.0 / 25.0;
@@ -152,43 +151,41 @@ PM_FN double pm_atan(double x)
 	p = p * y2 - 1.0 / 3.0;
 	p = p * y2 + 1.0;
 	res = off + y * p;
-	if (inv) res = PM_PI_2 - res;
+	res = inv ? PM_PI_2 - res : res;
 	return x < 0.0 ? -res : res;
 }
 
-/* cos(t) for |t| <= pi/2 */
+/* cos(t) for |t| <= pi/2: Taylor series of cos on [0, pi/4], of sin of the
+ * complement above; both are evaluated and one is selected */
 PM_FN double pm_cos(double t)
 {
-	double a = t < 0.0 ? -t : t, a2, p;
-	if (a <= 0.78539816339744830962) {
-		a2 = a * a;
-		/* 1 - a^2/2! + ... + a^20/20! */
-		p = 1.0 / 2432902008176640000.0;
-		p = -p * a2 + 1.0 / 6402373705728000.0;
-		p = -p * a2 + 1.0 / 20922789888000.0;
-		p = -p * a2 + 1.0 / 87178291200.0;
-		p = -p * a2 + 1.0 / 479001600.0;
-		p = -p * a2 + 1.0 / 3628800.0;
-		p = -p * a2 + 1.0 / 40320.0;
-		p = -p * a2 + 1.0 / 720.0;
-		p = -p * a2 + 1.0 / 24.0;
-		p = -p * a2 + 0.5;
-		return 1.0 - p * a2;
-	}
-	a = (PM_PI_2_HI - a) + PM_PI_2_LO; /* sin of the complement; the subtraction is exact */
-	a2 = a * a;
-	/* a - a^3/3! + ... + a^21/21! */
-	p = 1.0 / 51090942171709440000.0;
-	p = -p * a2 + 1.0 / 121645100408832000.0;
-	p = -p * a2 + 1.0 / 355687428096000.0;
-	p = -p * a2 + 1.0 / 1307674368000.0;
-	p = -p * a2 + 1.0 / 6227020800.0;
-	p = -p * a2 + 1.0 / 39916800.0;
-	p = -p * a2 + 1.0 / 362880.0;
-	p = -p * a2 + 1.0 / 5040.0;
-	p = -p * a2 + 1.0 / 120.0;
-	p = -p * a2 + 1.0 / 6.0;
-	return a - a * (p * a2);
+	const double a = t < 0.0 ? -t : t;
+	const double b = (PM_PI_2_HI - a) + PM_PI_2_LO; /* the subtraction is exact */
+	const double a2 = a * a, b2 = b * b;
+	double p, q;
+	/* 1 - a^2/2! + ... + a^20/20! */
+	p = 1.0 / 2432902008176640000.0;
+	p = -p * a2 + 1.0 / 6402373705728000.0;
+	p = -p * a2 + 1.0 / 20922789888000.0;
+	p = -p * a2 + 1.0 / 87178291200.0;
+	p = -p * a2 + 1.0 / 479001600.0;
+	p = -p * a2 + 1.0 / 3628800.0;
+	p = -p * a2 + 1.0 / 40320.0;
+	p = -p * a2 + 1.0 / 720.0;
+	p = -p * a2 + 1.0 / 24.0;
+	p = -p * a2 + 0.5;
+	/* b - b^3/3! + ... + b^21/21! */
+	q = 1.0 / 51090942171709440000.0;
+	q = -q * b2 + 1.0 / 121645100408832000.0;
+	q = -q * b2 + 1.0 / 355687428096000.0;
+	q = -q * b2 + 1.0 / 1307674368000.0;
+	q = -q * b2 + 1.0 / 6227020800.0;
+	q = -q * b2 + 1.0 / 39916800.0;
+	q = -q * b2 + 1.0 / 362880.0;
+	q = -q * b2 + 1.0 / 5040.0;
+	q = -q * b2 + 1.0 / 120.0;
+	q = -q * b2 + 1.0 / 6.0;
+	return a <= 0.78539816339744830962 ? 1.0 - p * a2 : b - b * (q * b2);
 }
 
 #endif

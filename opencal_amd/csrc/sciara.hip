@@ -145,6 +145,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 		const double sz0 = z[c];
 		int nb[9];
 		double zz[9], hh[9], H[9], th[9];
+		th[0] = 0.0;
 #pragma unroll
 		for (int k = 0; k < 9; ++k) {
 			nb[k] = k == 0 ? c : nb_flat(i, j, k, rows, cols);
@@ -152,18 +153,23 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 			const double zk = nb[k] >= 0 ? z[nb[k]] : 0.0;
 			zz[k] = k < 5 ? zk : sz0 - (sz0 - zk) / P.rad2;
 			H[k] = 0.0;
-			th[k] = 0.0;
 		}
 		H[0] = zz[0];
 		unsigned keep = 1u;
 		const double lvl0 = zz[0] + hh[0];
+		// the slope angles and critical thicknesses of all eight neighbours
+		// are computed without branches (eight independent chains the
+		// scheduler interleaves); only those of lower neighbours are used
+		double hcc[9];
 #pragma unroll
-		for (int k = 1; k < 9; ++k)
-			if (lvl0 > zz[k] + hh[k]) {
-				H[k] = zz[k] + hh[k];
-				th[k] = pm_atan((lvl0 - (zz[k] + hh[k])) / P.Pc);
-				keep |= 1u << k;
-			}
+		for (int k = 1; k < 9; ++k) {
+			const double lk = zz[k] + hh[k];
+			const bool down = lvl0 > lk;
+			H[k] = down ? lk : 0.0;
+			th[k] = pm_atan((lvl0 - lk) / P.Pc);
+			hcc[k] = hc * pm_cos(th[k]);
+			keep |= (down ? 1u : 0u) << k;
+		}
 		double avg;
 		bool loop;
 		do {
@@ -188,7 +194,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 #pragma unroll
 		for (int k = 1; k < 9; ++k) {
 			if (!((keep >> k) & 1u)) continue;
-			if (!(h0 > hc * pm_cos(th[k]))) continue;
+			if (!(h0 > hcc[k])) continue;
 			const double fk = Pr * (avg - H[k]);
 			if (fk > 0) {
 				f.f[k - 1][c] = fk;
@@ -277,6 +283,8 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_finish(double* __restrict__ h, do
 	const size_t total = sweep_total(cnt, n);
 	const size_t stride = (size_t)gridDim.x * blockDim.x;
 	const int lane = threadIdx.x & (dev::WAVE - 1);
+	__shared__ int wave_off[SC_BLOCK / dev::WAVE];
+	__shared__ int block_pos;
 	for (size_t base = (size_t)blockIdx.x * blockDim.x; base < total; base += stride) {
 		const size_t idx = base + threadIdx.x;
 		const bool valid = idx < total;
@@ -292,14 +300,23 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_finish(double* __restrict__ h, do
 			if (STEER && mb) h[c] = 0.0;
 		}
 		if (ACTIVE) {
+			// one atomic per block: waves' survivor counts scanned in LDS
 			const unsigned long long m = __ballot(surv);
-			if (m) {
-				const int leader = __ffsll((long long)m) - 1;
-				int pos = 0;
-				if (lane == leader) pos = atomicAdd(&count[2], __popcll(m));
-				pos = __shfl(pos, leader);
-				if (surv) alt[pos + __popcll(m & ((1ull << lane) - 1ull))] = c;
+			const int wid = threadIdx.x >> 6;
+			if (lane == 0) wave_off[wid] = __popcll(m);
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				int tot = 0;
+				for (int w = 0; w < SC_BLOCK / dev::WAVE; ++w) {
+					const int k = wave_off[w];
+					wave_off[w] = tot;
+					tot += k;
+				}
+				block_pos = tot ? atomicAdd(&count[2], tot) : 0;
 			}
+			__syncthreads();
+			if (surv) alt[block_pos + wave_off[wid] + __popcll(m & ((1ull << lane) - 1ull))] = c;
+			__syncthreads();
 		}
 	}
 	if (!ACTIVE) return;
@@ -541,7 +558,7 @@ struct SciaraProgram : Program {
 	{
 		if (!active) return clamp_grid(m->cells(), 1, 8192);
 		// room for growth between read-backs; the kernels loop over any excess
-		return clamp_grid((size_t)(2 * est + 4096), 16, 8192);
+		return clamp_grid((size_t)(est + est / 4 + 1024), 8, 8192);
 	}
 
 	int transition(ModelX* m, bool steer)

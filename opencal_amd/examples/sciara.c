@@ -82,7 +82,7 @@ static void power_law(CALreal sol, CALreal vent, CALreal* k1, CALreal* k2)
 }
 
 /* simulationInitialize (Sciara.cpp:254-285) */
-static void sciara_init(struct CALModel2D* ca)
+static void sciara_simulation_init(struct CALModel2D* ca)
 {
 	int v;
 	SC->elapsed_time = 0;
@@ -161,7 +161,7 @@ void* sciara_create(int rows, int cols, const double* z0, double cell_size, cons
 	calAddElementaryProcess2D(s->model, sciara_temperature);
 	if (active) calAddElementaryProcess2D(s->model, sciara_remove);
 	s->run = calRunDef2D(s->model, 1, steps, CAL_UPDATE_IMPLICIT);
-	calRunAddInitFunc2D(s->run, sciara_init);
+	calRunAddInitFunc2D(s->run, sciara_simulation_init);
 	calRunAddSteeringFunc2D(s->run, sciara_steering);
 	calRunAddStopConditionFunc2D(s->run, sciara_stop);
 
@@ -242,6 +242,32 @@ int sciara_run(void* handle)
 	calRun2D(s->run);
 	return calhipGetLastError() ? -1 : s->run->step;
 }
+
+/* calRunInitSimulation2D alone (for callers that step the model themselves) */
+int sciara_init(void* handle)
+{
+	sciara_t* s = (sciara_t*)handle;
+	SC = s;
+	calRunInitSimulation2D(s->run);
+	return calhipGetLastError() ? -1 : 0;
+}
+
+/* up to n calRunCAStep2D calls; stops early when the stop condition holds.
+ * Returns the number of steps run, -1 on error. */
+int sciara_steps(void* handle, int n)
+{
+	sciara_t* s = (sciara_t*)handle;
+	int k;
+	SC = s;
+	for (k = 0; k < n; ++k) {
+		CALbyte again = calRunCAStep2D(s->run);
+		if (calhipGetLastError()) return -1;
+		if (!again) return k + 1;
+	}
+	return n;
+}
+
+int sciara_sync(void* handle) { return calhipSynchronize2D(((sciara_t*)handle)->model); }
 
 /* which: 0 altitude, 1 lava thickness, 2 temperature, 3 solidified lava */
 int sciara_read(void* handle, int which, double* out)

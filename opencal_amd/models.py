@@ -213,6 +213,18 @@ class Sciara:
         self.steps_run = r
         return self
 
+    def init(self):
+        check(self.E.sciara_init(self.handle), "sciara_init")
+
+    def steps(self, n):
+        """Up to n steps (calRunCAStep2D each); returns the number run."""
+        r = self.E.sciara_steps(self.handle, int(n))
+        check(-1 if r < 0 else 0, "sciara_steps")
+        return r
+
+    def sync(self):
+        check(self.E.sciara_sync(self.handle), "sciara_sync")
+
     def read(self):
         out = {}
         for k, name in enumerate(("z", "h", "T", "Msl")):
