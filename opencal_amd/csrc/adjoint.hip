@@ -27,6 +27,7 @@
 // O(sqrt(steps)) grids and the forward is recomputed once.
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "engine.hpp"
@@ -40,7 +41,6 @@ namespace {
 using namespace sct;
 
 constexpr int ADJ_BLOCK = 256;
-constexpr int ADJ_RPW = 8;
 
 struct Box {
 	int r0, c0, nr, nc;   // dL/dz is accumulated for cells inside this box
@@ -142,54 +142,31 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_step(const double* __restri
 	}
 }
 
-// One forward step with the fused sweep (toroidal, any column count).
-__global__ __launch_bounds__(64 * FUSED_WAVES) void fwd_step(const double* __restrict__ z,
-                                                               const double* __restrict__ h,
-                                                               double* __restrict__ hn, double* __restrict__ trash,
-                                                               int rows, int cols, int n_strips, int n_logical,
-                                                               int per_xcd, double r, double eps)
-{
-	const int L = dev::xcd_logical_block(blockIdx.x, per_xcd);
-	if (L >= n_logical) return;
-	const int n_sgroups = (n_strips + FUSED_WAVES - 1) / FUSED_WAVES;
-	const int strip = (L % n_sgroups) * FUSED_WAVES + (threadIdx.x >> 6);
-	const int chunk = L / n_sgroups;
-	if (strip >= n_strips) return;
-	const int r0 = chunk * ADJ_RPW;
-	if (r0 >= rows) return;
-	if (cols % 2 == 0)
-		sweep_chunk<1, true>(z, h, hn, trash, rows, cols, strip, chunk, r0, min(r0 + ADJ_RPW, rows), r, eps);
-	else
-		sweep_chunk<1, false>(z, h, hn, trash, rows, cols, strip, chunk, r0, min(r0 + ADJ_RPW, rows), r, eps);
-}
-
-struct Fwd {
-	int n_strips, n_logical, per_xcd, blocks;
-	Fwd(int rows, int cols)
+// Work buffers kept for the life of the process and grown on demand, so that
+// repeated forward / gradient calls (an optimisation loop) do not pay for
+// device allocation every time.  Calls are serialised by the lock they hold
+// for their whole duration.
+struct WorkCache {
+	std::mutex mu;
+	void* p[5] = {};
+	size_t cap[5] = {};
+	void* get(int slot, size_t bytes)
 	{
-		n_strips = (cols + STRIP2_OUT - 1) / STRIP2_OUT;
-		int sgroups = (n_strips + FUSED_WAVES - 1) / FUSED_WAVES;
-		n_logical = sgroups * ((rows + ADJ_RPW - 1) / ADJ_RPW);
-		per_xcd = (n_logical + 7) / 8;
-		blocks = per_xcd * 8;
+		if (cap[slot] >= bytes) return p[slot];
+		if (p[slot]) (void)hipFree(p[slot]);
+		p[slot] = nullptr;
+		cap[slot] = 0;
+		if (CALHIP_CHECK(hipMalloc(&p[slot], bytes))) return nullptr;
+		cap[slot] = bytes;
+		return p[slot];
 	}
 };
-
-struct Scoped {
-	void* p = nullptr;
-	~Scoped()
-	{
-		if (p) (void)hipFree(p);
-	}
-};
-
-int launch_fwd(hipStream_t st, const Fwd& g, const double* z, const double* h, double* hn, double* trash, int rows,
-               int cols, double r, double eps)
+WorkCache& work()
 {
-	hipLaunchKernelGGL(fwd_step, dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, st, z, h, hn, trash, rows, cols,
-	                   g.n_strips, g.n_logical, g.per_xcd, r, eps);
-	return 0;
+	static WorkCache* w = new WorkCache();   // never destroyed: freed with the process
+	return *w;
 }
+enum { W_A, W_B, W_TRASH, W_STASH, W_PART };
 
 bool check_args(int rows, int cols, int steps, int ck)
 {
@@ -215,22 +192,24 @@ extern "C" int calhipSciddicaTForwardDev(int rows, int cols, const double* d_z, 
 	const size_t n = (size_t)rows * cols, bytes = n * sizeof(double);
 	hipStream_t st = nullptr;
 	if (CALHIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking))) return -1;
-	Scoped bufA, bufB, trash;
+	WorkCache& W = work();
+	std::lock_guard<std::mutex> lk(W.mu);
 	int rc = -1;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	do {
-		if (CALHIP_CHECK(hipMalloc(&bufA.p, bytes)) || CALHIP_CHECK(hipMalloc(&bufB.p, bytes)) ||
-		    CALHIP_CHECK(hipMalloc(&trash.p, 128 * sizeof(double))))
-			break;
+		void* bufA = W.get(W_A, bytes);
+		void* bufB = W.get(W_B, bytes);
+		double* trash = (double*)W.get(W_TRASH, 128 * sizeof(double));
+		if (!bufA || !bufB || !trash) break;
 		if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
-		Fwd g(rows, cols);
-		double* b[2] = {(double*)bufA.p, (double*)bufB.p};
+		double* b[2] = {(double*)bufA, (double*)bufB};
+		const int rpw = sweep_rpw(st, rows, cols, d_z, d_h0, b[1], trash, p_r, p_eps);
 		if (CALHIP_CHECK(hipMemcpyAsync(b[0], d_h0, bytes, hipMemcpyDeviceToDevice, st))) break;
 		if (CALHIP_CHECK(hipEventRecord(e0, st))) break;
 		for (int t = 0; t < steps; ++t) {
 			if (t % ck == 0 && d_ckpt)
 				(void)hipMemcpyAsync(d_ckpt + (size_t)(t / ck) * n, b[t & 1], bytes, hipMemcpyDeviceToDevice, st);
-			launch_fwd(st, g, d_z, b[t & 1], b[(t + 1) & 1], (double*)trash.p, rows, cols, p_r, p_eps);
+			launch_sweep(st, rows, cols, d_z, b[t & 1], b[(t + 1) & 1], trash, p_r, p_eps, rpw);
 		}
 		if (steps % ck == 0 && d_ckpt)
 			(void)hipMemcpyAsync(d_ckpt + (size_t)(steps / ck) * n, b[steps & 1], bytes, hipMemcpyDeviceToDevice, st);
@@ -261,35 +240,37 @@ static int adjoint_impl(int rows, int cols, const double* d_z, const double* d_c
 	const int nblk = (int)((n + ADJ_BLOCK - 1) / ADJ_BLOCK);
 	hipStream_t st = nullptr;
 	if (CALHIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking))) return -1;
-	Scoped stash, lamB, part, trash;
+	WorkCache& W = work();
+	std::lock_guard<std::mutex> lk(W.mu);
 	int rc = -1;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	do {
 		const int seg_len = ck < steps ? ck : (steps > 0 ? steps : 1);
-		if (CALHIP_CHECK(hipMalloc(&stash.p, bytes * (size_t)(seg_len + 1))) ||
-		    CALHIP_CHECK(hipMalloc(&lamB.p, bytes)) ||
-		    CALHIP_CHECK(hipMalloc(&part.p, sizeof(double) * 2 * (size_t)nblk)) ||
-		    CALHIP_CHECK(hipMalloc(&trash.p, 128 * sizeof(double))))
-			break;
-		if (CALHIP_CHECK(hipMemsetAsync(part.p, 0, sizeof(double) * 2 * (size_t)nblk, st))) break;
+		double* stash = (double*)W.get(W_STASH, bytes * (size_t)(seg_len + 1));
+		double* lamB = (double*)W.get(W_A, bytes);
+		double* partp = (double*)W.get(W_PART, sizeof(double) * 2 * (size_t)nblk);
+		double* trash = (double*)W.get(W_TRASH, 128 * sizeof(double));
+		if (!stash || !lamB || !partp || !trash) break;
+		if (CALHIP_CHECK(hipMemsetAsync(partp, 0, sizeof(double) * 2 * (size_t)nblk, st))) break;
 		if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
 		if (CALHIP_CHECK(hipEventRecord(e0, st))) break;
-		Fwd g(rows, cols);
-		double* lam[2] = {d_lam, (double*)lamB.p};
+		// the tuner's timing sweeps write into the stash, refilled below
+		const int rpw = sweep_rpw(st, rows, cols, d_z, d_ckpt, stash, trash, p_r, p_eps);
+		double* lam[2] = {d_lam, lamB};
 		int cur = 0;   // lam[cur] holds dL/dh at the current time
-		double* S = (double*)stash.p;
+		double* S = stash;
 		const int nseg = (steps + ck - 1) / ck;
 		for (int sgi = nseg - 1; sgi >= 0; --sgi) {
 			const int t0 = sgi * ck, t1 = t0 + ck < steps ? t0 + ck : steps;
 			// recompute h_t0 .. h_{t1-1} into the stash
 			if (CALHIP_CHECK(hipMemcpyAsync(S, d_ckpt + (size_t)sgi * n, bytes, hipMemcpyDeviceToDevice, st))) break;
 			for (int t = t0; t < t1 - 1; ++t)
-				launch_fwd(st, g, d_z, S + (size_t)(t - t0) * n, S + (size_t)(t - t0 + 1) * n, (double*)trash.p, rows,
-				           cols, p_r, p_eps);
+				launch_sweep(st, rows, cols, d_z, S + (size_t)(t - t0) * n, S + (size_t)(t - t0 + 1) * n, trash, p_r,
+				             p_eps, rpw);
 			for (int t = t1 - 1; t >= t0; --t) {
 				hipLaunchKernelGGL(adjoint_step, dim3(nblk), dim3(ADJ_BLOCK), 0, st, d_z,
 				                   (const double*)(S + (size_t)(t - t0) * n), (const double*)lam[cur], lam[cur ^ 1],
-				                   d_lam_z, (double*)part.p, rows, cols, p_r, p_eps, box);
+				                   d_lam_z, partp, rows, cols, p_r, p_eps, box);
 				cur ^= 1;
 			}
 		}
@@ -297,7 +278,7 @@ static int adjoint_impl(int rows, int cols, const double* d_z, const double* d_c
 		if (CALHIP_CHECK(hipEventRecord(e1, st))) break;
 		if (cur == 1 && CALHIP_CHECK(hipMemcpyAsync(d_lam, lam[1], bytes, hipMemcpyDeviceToDevice, st))) break;
 		std::vector<double> hp(2 * (size_t)nblk);
-		if (CALHIP_CHECK(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * 2 * (size_t)nblk, hipMemcpyDeviceToHost,
+		if (CALHIP_CHECK(hipMemcpyAsync(hp.data(), partp, sizeof(double) * 2 * (size_t)nblk, hipMemcpyDeviceToHost,
 		                                st)))
 			break;
 		if (CALHIP_CHECK(hipStreamSynchronize(st))) break;

@@ -10,15 +10,19 @@
 //   it (:94-100).
 //
 // Kernels:
-//   step_fused    full grid, both processes in one sweep.  Each wave owns a
-//                 64-column strip (60 output columns plus a 2-column halo on
-//                 each side) and walks down a chunk of rows with a three-row
-//                 register window of z and h; horizontal neighbours come from
-//                 the adjacent lanes.  The outflows of a cell are computed once,
+//   step_fused2   full grid, both processes in one sweep (sweep_strip in
+//                 sciddicat_device.hpp).  Each wave owns a 128-column strip
+//                 (two cells per lane, 124 output columns plus a 2-column halo
+//                 on each side) and walks a chunk of 8 rows with a register
+//                 ring of z and h rows; horizontal neighbours come from the
+//                 adjacent lanes.  The outflows of a cell are computed once,
 //                 consumed by the width update of its four neighbours through
-//                 lane shuffles (east/west) and the window (north/south), and
-//                 never written to memory.  HBM traffic per cell: z and h read,
-//                 h written = 24 B.
+//                 lane shuffles (east/west) and the ring (north/south), and
+//                 never written to memory.  Neighbouring chunks walk in
+//                 opposite directions so the halo rows they share are read
+//                 close together in time (from L2), and blocks are remapped so
+//                 that each XCD sweeps one contiguous band of the grid.  HBM
+//                 traffic per cell: z and h read, h written = 24 B.
 //   flows_full / width_full
 //                 one process per sweep with the flows written out, the exact
 //                 reference sequence for models whose flows are observed.
@@ -168,26 +172,6 @@ int stride_grid(size_t n, int block)
 
 }  // namespace
 
-// Launch geometry of the fused kernel for a rows x cols grid.
-struct FusedGeom {
-	int n_strips, n_groups, n_logical, per_xcd, blocks, rpw;
-};
-
-// Two-column kernel: a logical block is FUSED_WAVES adjacent strips of one
-// row chunk.
-static FusedGeom fused_geom2(int rows, int cols, int rpw)
-{
-	FusedGeom g;
-	g.rpw = rpw;
-	g.n_strips = (cols + STRIP2_OUT - 1) / STRIP2_OUT;
-	int sgroups = (g.n_strips + FUSED_WAVES - 1) / FUSED_WAVES;
-	g.n_groups = (rows + rpw - 1) / rpw;
-	g.n_logical = sgroups * g.n_groups;
-	g.per_xcd = (g.n_logical + 7) / 8;
-	g.blocks = g.per_xcd * 8;
-	return g;
-}
-
 // ------------------------------------------------------------------ program
 
 struct SciddicaTProgram : Program {
@@ -197,7 +181,7 @@ struct SciddicaTProgram : Program {
 	const CALreal* p_r = nullptr;
 	const CALreal* p_eps = nullptr;
 	bool active = false;
-	int rpw_override = 0;
+	int rpw = 0;   // row-chunk length of the fused sweep (0: not chosen yet)
 	bool h_prev_valid = false;   // h->d_nxt still holds the pre-step thickness
 
 	const char* name() const override { return "SciddicaT"; }
@@ -252,12 +236,18 @@ struct SciddicaTProgram : Program {
 	int launch2(ModelX* m, double r, double eps)
 	{
 		const int rows = m->pub.rows, cols = m->pub.columns;
-		// short row chunks: the few waves that meet wet cells run the
-		// minimisation on every row of their chunk, so long chunks make them
-		// the critical path of the sweep
-		int rpw = rpw_override > 0 ? rpw_override : 8;
-		FusedGeom g = fused_geom2(rows, cols, rpw);
 		if (!d_trash && CALHIP_CHECK(hipMalloc(&d_trash, 128 * sizeof(double)))) return -1;
+		// row-chunk length: autotuned per grid shape (the sweeps it times
+		// write into h's next layer, which this step overwrites)
+		if (rpw == 0)
+			rpw = sweep_rpw(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
+			                (double*)h->d_nxt, d_trash, r, eps);
+		if (D == 1) {
+			launch_sweep(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
+			             (double*)h->d_nxt, d_trash, r, eps, rpw);
+			return 0;
+		}
+		const SweepGeom g = sweep_geom(rows, cols, rpw);
 		if (cols % 2 == 0)
 			hipLaunchKernelGGL((step_fused2<D, true>), dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, m->stream,
 			                   (const double*)z->d_cur, (const double*)h->d_cur, (double*)h->d_nxt, d_trash, rows,
@@ -525,7 +515,6 @@ extern "C" int calhipBindSciddicaT2D(struct CALModel2D* ca2D, CALCallbackFunc2D 
 	p->p_r = p_r;
 	p->p_eps = p_epsilon;
 	p->active = ca2D->OPTIMIZATION == CAL_OPT_ACTIVE_CELLS;
-	if (const char* e = getenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE")) p->rpw_override = atoi(e);
 	if (const char* e = getenv("CALHIP_SCIDDICAT_PREFETCH")) p->prefetch = atoi(e);
 	delete m->prog;
 	m->prog = p;

@@ -292,6 +292,21 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) void step_fused2(
 	sweep_chunk<D, EVEN>(z, h, hn, trash, rows, cols, strip, chunk, r0, min(r0 + rpw, rows), r, eps);
 }
 
+// Launch shape of step_fused2 for a rows x cols grid with row chunks of rpw
+// (sweep_tune.hip).
+struct SweepGeom {
+	int n_strips, n_logical, per_xcd, blocks, rpw;
+};
+SweepGeom sweep_geom(int rows, int cols, int rpw);
+// One fused step h -> hn (prefetch depth 1) on stream st.
+void launch_sweep(hipStream_t st, int rows, int cols, const double* z, const double* h, double* hn, double* trash,
+                  double r, double eps, int rpw);
+// The row-chunk length for this grid shape: the environment's
+// CALHIP_SCIDDICAT_ROWS_PER_WAVE, else the autotuned one (timed on first use
+// by writing sweeps of h into `scratch`, then cached per shape).
+int sweep_rpw(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch, double* trash,
+              double r, double eps);
+
 }  // namespace sct
 }  // namespace calhip
 
