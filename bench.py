@@ -75,6 +75,22 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024):
     }
 
 
+def pcie_inclusive_sciddicat(n, steps, seed=2024):
+    """The same workload as one calRun2D call on host arrays: the model's init
+    on the host, upload of every substate, the steps, download at the end."""
+    from opencal_amd import synth
+    from opencal_amd.models import SciddicaT
+    z, h, _ = synth.sciddicat_inputs(n, n, seed)
+    m = SciddicaT(z, h, 0.5, 0.001, steps, active=False)
+    t0 = time.perf_counter()
+    m.run()
+    m.read()
+    wall = time.perf_counter() - t0
+    m.close()
+    return {"gcups": round(n * n * steps / wall / 1e9, 3), "wall_s": round(wall, 4),
+            "note": "calRun2D from host arrays incl. host init, upload of 6 substates x 2 layers and read-back"}
+
+
 def cpu_baseline_sciddicat(n, steps, seed=2024):
     import native
     from opencal_amd import synth
@@ -275,7 +291,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--only", choices=["active", "life", "batch", "adjoint", "sciara"],
@@ -317,6 +333,8 @@ def main():
                      "kernel": "step_fused2", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
                      "bytes_per_cell": BYTES_PER_CELL_SCIDDICAT},
     }
+    log("pcie-inclusive run ...")
+    out["pcie_inclusive"] = pcie_inclusive_sciddicat(args.size, args.steps)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline_sciddicat(args.size, args.cpu_steps)

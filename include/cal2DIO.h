@@ -1,4 +1,5 @@
-/* Substate text IO (OpenCAL-OMP/include/cal2DIO.h:21-71, cal2DBufferIO.h:22-72).
+/* Substate text IO (OpenCAL-OMP/include/cal2DIO.h:21-71); the matrix
+ * functions of cal2DBufferIO.h come with it.
  * Whitespace-separated matrices; reals are written with "%.9f" as the
  * OpenMP reference does (OpenCAL-OMP/source/cal2DBufferIO.c:142).  The save
  * functions first bring the substate back from the GPU when a device run
@@ -8,23 +9,11 @@
 
 #include <stdio.h>
 #include <cal2D.h>
+#include <cal2DBufferIO.h>
 
 #ifdef __cplusplus
 extern "C" {
 #endif
-
-void calfLoadMatrix2Db(CALbyte* M, int rows, int columns, FILE* f);
-void calfLoadMatrix2Di(CALint* M, int rows, int columns, FILE* f);
-void calfLoadMatrix2Dr(CALreal* M, int rows, int columns, FILE* f);
-CALbyte calLoadMatrix2Db(CALbyte* M, int rows, int columns, char* path);
-CALbyte calLoadMatrix2Di(CALint* M, int rows, int columns, char* path);
-CALbyte calLoadMatrix2Dr(CALreal* M, int rows, int columns, char* path);
-void calfSaveMatrix2Db(CALbyte* M, int rows, int columns, FILE* f);
-void calfSaveMatrix2Di(CALint* M, int rows, int columns, FILE* f);
-void calfSaveMatrix2Dr(CALreal* M, int rows, int columns, FILE* f);
-CALbyte calSaveMatrix2Db(CALbyte* M, int rows, int columns, char* path);
-CALbyte calSaveMatrix2Di(CALint* M, int rows, int columns, char* path);
-CALbyte calSaveMatrix2Dr(CALreal* M, int rows, int columns, char* path);
 
 void calfLoadSubstate2Db(struct CALModel2D* ca2D, struct CALSubstate2Db* Q, FILE* f);
 void calfLoadSubstate2Di(struct CALModel2D* ca2D, struct CALSubstate2Di* Q, FILE* f);
