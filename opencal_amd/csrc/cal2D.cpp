@@ -363,8 +363,10 @@ static void init_sub(CALModel2D* m, void* q, T v, bool both)
 {
 	ModelX* x = X(m);
 	SubX* s = S(q);
+	// a host-only substate of an active-cells model needs the current list
+	if (s->host_only && x->active() && ensure_host(x)) return;
 	if (!m->A.cells && !(x->phase == Phase::Device && x->active())) v = full_grid_value<T>(v);
-	if (x->phase == Phase::Device) {
+	if (x->phase == Phase::Device && !s->host_only) {
 		device_fill(x, s, (double)v, both);
 		return;
 	}
@@ -399,7 +401,7 @@ static void update_one(CALModel2D* m, void* q)
 {
 	ModelX* x = X(m);
 	SubX* s = S(q);
-	if (x->phase == Phase::Device) {
+	if (x->phase == Phase::Device && !s->host_only) {
 		if (s->nxt_dirty) flip(s);
 		return;
 	}
@@ -499,28 +501,28 @@ static inline size_t nb_index(CALModel2D* m, int i, int j, int n)
 #define CAL_ACCESSORS(SUF, T)                                                                       \
 	extern "C" void calInit2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, T v) \
 	{                                                                                                \
-		if (ensure_host(X(m))) return;                                                               \
+		if (ensure_host_for(X(m), Q)) return;                                                        \
 		cur_of<T>(Q)[idx(m, i, j)] = v;                                                              \
 		nxt_of<T>(Q)[idx(m, i, j)] = v;                                                              \
 	}                                                                                                \
 	extern "C" T calGet2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j)     \
 	{                                                                                                \
-		if (ensure_host(X(m))) return (T)0;                                                          \
+		if (ensure_host_for(X(m), Q)) return (T)0;                                                   \
 		return cur_of<T>(Q)[idx(m, i, j)];                                                           \
 	}                                                                                                \
 	extern "C" T calGetX2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, int n) \
 	{                                                                                                \
-		if (ensure_host(X(m))) return (T)0;                                                          \
+		if (ensure_host_for(X(m), Q)) return (T)0;                                                   \
 		return cur_of<T>(Q)[nb_index(m, i, j, n)];                                                   \
 	}                                                                                                \
 	extern "C" void calSet2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, T v) \
 	{                                                                                                \
-		if (ensure_host(X(m))) return;                                                               \
+		if (ensure_host_for(X(m), Q)) return;                                                        \
 		nxt_of<T>(Q)[idx(m, i, j)] = v;                                                              \
 	}                                                                                                \
 	extern "C" void calSetCurrent2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, T v) \
 	{                                                                                                \
-		if (ensure_host(X(m))) return;                                                               \
+		if (ensure_host_for(X(m), Q)) return;                                                        \
 		cur_of<T>(Q)[idx(m, i, j)] = v;                                                              \
 	}
 

@@ -65,25 +65,25 @@ extern "C" void calfSaveMatrix2Dr(CALreal* M, int rows, int columns, FILE* f) { 
 	}                                                                                       \
 	extern "C" void calfLoadSubstate2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, FILE* f) \
 	{                                                                                       \
-		if (ensure_host(X(m))) return;                                                      \
+		if (ensure_host_for(X(m), Q)) return;                                                      \
 		calfLoadMatrix2D##SUF(Q->current, m->rows, m->columns, f);                          \
 		if (Q->next) memcpy(Q->next, Q->current, sizeof(T) * (size_t)m->rows * m->columns); \
 	}                                                                                       \
 	extern "C" CALbyte calLoadSubstate2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, char* path) \
 	{                                                                                       \
-		if (ensure_host(X(m))) return CAL_FALSE;                                            \
+		if (ensure_host_for(X(m), Q)) return CAL_FALSE;                                            \
 		CALbyte ok = calLoadMatrix2D##SUF(Q->current, m->rows, m->columns, path);           \
 		if (Q->next) memcpy(Q->next, Q->current, sizeof(T) * (size_t)m->rows * m->columns); \
 		return ok;                                                                          \
 	}                                                                                       \
 	extern "C" void calfSaveSubstate2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, FILE* f) \
 	{                                                                                       \
-		if (ensure_host(X(m))) return;                                                      \
+		if (ensure_host_for(X(m), Q)) return;                                                      \
 		calfSaveMatrix2D##SUF(Q->current, m->rows, m->columns, f);                          \
 	}                                                                                       \
 	extern "C" CALbyte calSaveSubstate2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, char* path) \
 	{                                                                                       \
-		if (ensure_host(X(m))) return CAL_FALSE;                                            \
+		if (ensure_host_for(X(m), Q)) return CAL_FALSE;                                            \
 		return calSaveMatrix2D##SUF(Q->current, m->rows, m->columns, path);                 \
 	}
 

@@ -102,6 +102,10 @@ struct SubX {
 	bool nxt_stale = false;
 	bool nxt_dirty = false;
 	Transient tstate = Transient::None;
+	// Never moved to the device: a single-layer substate the bound program
+	// does not use.  Its host array stays authoritative in every phase, so
+	// user code may even replace or free it as OpenCAL code does.
+	bool host_only = false;
 
 	void* h_cur() const { return (void*)pub.r.current; }
 	void* h_nxt() const { return (void*)pub.r.next; }
@@ -151,6 +155,8 @@ int check_hip(hipError_t e, const char* what);
 
 // ---- residency ----
 int ensure_host(ModelX* m);    // DEVICE -> HOST (no-op in HOST)
+// ensure_host, unless the call touches only the host-only substate q
+inline int ensure_host_for(ModelX* m, void* q) { return S(q)->host_only ? 0 : ensure_host(m); }
 int ensure_device(ModelX* m);  // HOST -> DEVICE (no-op in DEVICE)
 int ensure_device_buffers(ModelX* m, SubX* s);
 int materialise_next(ModelX* m, SubX* s);  // stale next := current

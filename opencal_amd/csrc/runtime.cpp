@@ -49,6 +49,7 @@ int ensure_device(ModelX* m)
 	CALModel2D* p = &m->pub;
 	size_t n = m->cells();
 	for (SubX* s : m->subs) {
+		if (s->host_only) continue;
 		if (ensure_device_buffers(m, s)) return -1;
 		size_t bytes = n * (size_t)s->elem;
 		s->tstate = Transient::None;
@@ -103,12 +104,14 @@ int ensure_host(ModelX* m)
 	size_t n = m->cells();
 	if (m->prog && m->prog->sync_active(m)) return -1;
 	for (SubX* s : m->subs) {
+		if (s->host_only) continue;
 		if (s->tstate != Transient::None && s->tstate != Transient::Zero && m->prog) {
 			if (m->prog->materialise(m, s)) return -1;
 			m->stats.materialisations++;
 		}
 	}
 	for (SubX* s : m->subs) {
+		if (s->host_only) continue;
 		size_t bytes = n * (size_t)s->elem;
 		if (s->tstate == Transient::Zero) {
 			memset(s->h_cur(), 0, bytes);
@@ -131,7 +134,7 @@ int ensure_host(ModelX* m)
 	}
 	if (CALHIP_CHECK(hipStreamSynchronize(m->stream))) return -1;
 	for (SubX* s : m->subs)
-		if (!s->single && s->nxt_stale && s->tstate != Transient::Zero)
+		if (!s->host_only && !s->single && s->nxt_stale && s->tstate != Transient::Zero)
 			memcpy(s->h_nxt(), s->h_cur(), n * (size_t)s->elem);
 	if (m->active()) {
 		lin.resize((size_t)cnt + 1);
@@ -221,7 +224,7 @@ int device_update(ModelX* m)
 {
 	if (device_rebuild_active(m)) return -1;
 	for (SubX* s : m->subs)
-		if (!s->single && s->nxt_dirty) flip(s);
+		if (!s->host_only && !s->single && s->nxt_dirty) flip(s);
 	if (m->prog) m->prog->transient_update(m);
 	return 0;
 }

@@ -709,6 +709,12 @@ extern "C" int calhipBindSciara2D(struct CALModel2D* ca2D, const struct calhipSc
 			set_error(34, "calhipBindSciara2D: a vent lies outside the grid");
 			return -1;
 		}
+	// single-layer substates the program does not use (the reference's Mv and
+	// Sz_t0) stay on the host: OpenCAL code replaces and frees their arrays
+	// (io.cpp loadVents / saveConfiguration)
+	if (m->phase == Phase::Device && ensure_host(m)) return -1;
+	for (SubX* s : m->subs)
+		s->host_only = s->single && s != S(b->Mb) && s != S(b->Msl);
 	auto* p = new SciaraProgram();
 	p->b = *b;
 	p->z = S(b->Sz);

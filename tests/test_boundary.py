@@ -214,6 +214,7 @@ def test_text_io_roundtrip(libs, tmp_path):
     text = open(p, "r").read()
     assert text.splitlines()[0].startswith("%.9f  %.9f" % (vals[0], vals[1]))
     q2 = mine.calAddSubstate2Dr(m)
+    mine.calLoadSubstate2Dr.restype = C.c_byte   # CALbyte
     assert mine.calLoadSubstate2Dr(m, q2, p) == 1
     assert abs(mine.calGet2Dr(m, q2, 4, 5) - vals[29]) < 1e-9
     if ref is not None:

@@ -31,3 +31,31 @@ def test_reference_program_runs_unchanged_on_the_engine(tmp_path):
     assert "Simulation terminated" in r.stdout
     want = gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
     assert (data / "width_final.txt").read_text() == want
+
+
+BIN_SC = os.path.join(native.REF_DIR, "dropin_sciara")
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_SC), reason="dropin_sciara not built (needs the reference tree)")
+def test_reference_sciara_program_on_the_engine(tmp_path):
+    """The reference SCIARA-fv2 program (main.cpp + its sources, one added bind
+    call) on its own 2006 data: 1000 steps, full grid, saved with "%.9f".  The
+    GPU's files equal, as text, the oracle run with the same portable
+    elementary functions, and lie within 1e-9 relative of the glibc run (which
+    equals the reference program on the CPU)."""
+    import numpy as np
+    import sciara2006
+    print("case dropin sciara 2006 data 378x517 1000 steps", flush=True)
+    sciara2006.write_inputs(str(tmp_path))
+    r = subprocess.run([BIN_SC], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    inp = sciara2006.load_inputs()
+    port = native.oracle_sciara(inp, 1000, False, portable=True)
+    libm = native.oracle_sciara(inp, 1000, False, portable=False)
+    for name, k, scale in (("Thickness", "h", 1.0), ("Temperature", "T", 1400.0), ("Morphology", "z", 3400.0),
+                           ("SolidifiedLavaThickness", "Msl", 1.0)):
+        path = tmp_path / "data" / "2006_SAVE" / f"{sciara2006.PREFIX}_{name}.stt"
+        text = path.read_text().split()[12:]
+        assert text == ["%.9f" % v for v in port[k].ravel()], name
+        got = np.array(text, dtype=np.float64)
+        assert np.abs(got - libm[k].ravel()).max() <= 1e-9 * scale + 1e-9, name

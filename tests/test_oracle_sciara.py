@@ -137,3 +137,29 @@ def test_portable_special_values():
     assert abs(_pm(1, np.array([1.0]))[0] - math.pi / 4) <= 2e-16
     assert _pm(0, np.array([0.0, 1.0, 2.0])).tolist() == [1.0, 10.0, 100.0]
     assert _pm(3, np.array([8.0, 27.0])).tolist() == pytest.approx([2.0, 3.0], rel=4e-16)
+
+
+SAVED = (("Thickness", "h"), ("Temperature", "T"), ("Morphology", "z"), ("SolidifiedLavaThickness", "Msl"))
+
+
+def _saved_text(run_dir, name):
+    import os
+    import sciara2006
+    path = os.path.join(run_dir, "data", "2006_SAVE", f"{sciara2006.PREFIX}_{name}.stt")
+    return open(path).read().split()[12:]   # after the 6 header lines
+
+
+@pytest.mark.skipif(not native.os.path.exists(native.os.path.join(native.REF_DIR, "calomp_sciara")),
+                    reason="reference program oracle/_ref/calomp_sciara not built")
+def test_oracle_reproduces_reference_program_on_its_data(tmp_path):
+    """The reference SCIARA-fv2 program itself (main.cpp: 2006 data, 1000 steps,
+    full grid, saved with "%.9f") against the oracle on the same files."""
+    import subprocess
+    import sciara2006
+    sciara2006.write_inputs(str(tmp_path))
+    r = subprocess.run([native.os.path.join(native.REF_DIR, "calomp_sciara")], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = native.oracle_sciara(sciara2006.load_inputs(), 1000, False, portable=False)
+    for name, k in SAVED:
+        assert _saved_text(str(tmp_path), name) == ["%.9f" % v for v in o[k].ravel()], name
