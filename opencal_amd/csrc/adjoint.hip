@@ -150,6 +150,7 @@ struct WorkCache {
 	std::mutex mu;
 	void* p[5] = {};
 	size_t cap[5] = {};
+	sct::WetOrder wo;   // wet-first order of the forward sweeps
 	void* get(int slot, size_t bytes)
 	{
 		if (cap[slot] >= bytes) return p[slot];
@@ -203,13 +204,13 @@ extern "C" int calhipSciddicaTForwardDev(int rows, int cols, const double* d_z, 
 		if (!bufA || !bufB || !trash) break;
 		if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
 		double* b[2] = {(double*)bufA, (double*)bufB};
-		const int rpw = sweep_rpw(st, rows, cols, d_z, d_h0, b[1], trash, p_r, p_eps);
+		const SweepShape shape = sweep_shape(st, rows, cols, d_z, d_h0, b[1], trash, p_r, p_eps);
 		if (CALHIP_CHECK(hipMemcpyAsync(b[0], d_h0, bytes, hipMemcpyDeviceToDevice, st))) break;
 		if (CALHIP_CHECK(hipEventRecord(e0, st))) break;
 		for (int t = 0; t < steps; ++t) {
 			if (t % ck == 0 && d_ckpt)
 				(void)hipMemcpyAsync(d_ckpt + (size_t)(t / ck) * n, b[t & 1], bytes, hipMemcpyDeviceToDevice, st);
-			launch_sweep(st, rows, cols, d_z, b[t & 1], b[(t + 1) & 1], trash, p_r, p_eps, rpw);
+			launch_sweep(st, rows, cols, d_z, b[t & 1], b[(t + 1) & 1], trash, p_r, p_eps, shape, &W.wo);
 		}
 		if (steps % ck == 0 && d_ckpt)
 			(void)hipMemcpyAsync(d_ckpt + (size_t)(steps / ck) * n, b[steps & 1], bytes, hipMemcpyDeviceToDevice, st);
@@ -255,7 +256,7 @@ static int adjoint_impl(int rows, int cols, const double* d_z, const double* d_c
 		if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
 		if (CALHIP_CHECK(hipEventRecord(e0, st))) break;
 		// the tuner's timing sweeps write into the stash, refilled below
-		const int rpw = sweep_rpw(st, rows, cols, d_z, d_ckpt, stash, trash, p_r, p_eps);
+		const SweepShape shape = sweep_shape(st, rows, cols, d_z, d_ckpt, stash, trash, p_r, p_eps);
 		double* lam[2] = {d_lam, lamB};
 		int cur = 0;   // lam[cur] holds dL/dh at the current time
 		double* S = stash;
@@ -266,7 +267,7 @@ static int adjoint_impl(int rows, int cols, const double* d_z, const double* d_c
 			if (CALHIP_CHECK(hipMemcpyAsync(S, d_ckpt + (size_t)sgi * n, bytes, hipMemcpyDeviceToDevice, st))) break;
 			for (int t = t0; t < t1 - 1; ++t)
 				launch_sweep(st, rows, cols, d_z, S + (size_t)(t - t0) * n, S + (size_t)(t - t0 + 1) * n, trash, p_r,
-				             p_eps, rpw);
+				             p_eps, shape, &W.wo);
 			for (int t = t1 - 1; t >= t0; --t) {
 				hipLaunchKernelGGL(adjoint_step, dim3(nblk), dim3(ADJ_BLOCK), 0, st, d_z,
 				                   (const double*)(S + (size_t)(t - t0) * n), (const double*)lam[cur], lam[cur ^ 1],

@@ -2,13 +2,17 @@
 // of uneven sizes and parameters stepped together.
 //
 // Every instance keeps its own z and double-buffered h in one packed device
-// allocation (256-byte aligned offsets).  The work of one step over all
-// instances is a single flat list of tiles (instance, group of 4 adjacent
-// 124-column strips, chunk of 8 rows); one launch per step covers the whole
-// ensemble, so the launch cost is paid once per step for all instances and
-// the GPU is filled even when each instance alone is far too small to fill
-// it.  The sweep of a tile is the fused two-process sweep of the single-model
-// program, so each instance's result equals its own plain run bit for bit.
+// allocation.  Rows of an instance with an even column count are padded to a
+// pitch of 16 columns, so every row starts on a 128-byte line and the aligned
+// 128-column strip sweep applies; odd column counts keep packed rows and the
+// 124-column strip sweep.  The work of one step over all instances is a
+// single flat list of wave items (instance, strip, chunk of rows), run four
+// to a block; one launch per step covers the whole ensemble, so the launch cost is
+// paid once per step for all instances and the GPU is filled even when each
+// instance alone is far too small to fill it.  The sweep of a tile is the
+// fused two-process sweep of the single-model program, so each instance's
+// result equals its own plain run bit for bit.
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -24,33 +28,45 @@ namespace {
 
 using namespace sct;
 
-constexpr int BATCH_RPW = 8;
+constexpr int BATCH_RPW = 16;   // rows per chunk
+constexpr int BATCH_DEPTH = 2;  // rows prefetched ahead by the aligned sweep
 
 struct BatchInst {
-	int rows, cols, strips, pad;
-	long long off;   // element offset of the instance's grids
+	int rows, cols, strips, pitch;   // pitch: row stride in elements
+	long long off;                   // element offset of the instance's grids
 	double r, eps;
 };
 
+// One step of every instance.  Wave items (instance, strip, chunk) form one
+// flat list; a block runs FUSED_WAVES consecutive items (so no wave idles at
+// an instance's last strip group), and blocks run in wet-first order (ws)
+// when given.
+template <int D>
 __global__ __launch_bounds__(64 * FUSED_WAVES) void batch_step(const BatchInst* __restrict__ inst,
                                                                  const int4* __restrict__ items, int n_items,
-                                                                 int per_xcd, const double* __restrict__ z,
+                                                                 int n_blocks, int per_xcd, int rpw,
+                                                                 const double* __restrict__ z,
                                                                  const double* __restrict__ h,
-                                                                 double* __restrict__ hn, double* __restrict__ trash)
+                                                                 double* __restrict__ hn, double* __restrict__ trash,
+                                                                 WetSets ws)
 {
-	const int L = dev::xcd_logical_block(blockIdx.x, per_xcd);
-	if (L >= n_items) return;
-	const int4 it = items[L];
-	const BatchInst I = inst[it.x];
-	const int strip = it.y * FUSED_WAVES + (threadIdx.x >> 6);
-	if (strip >= I.strips) return;
-	const int r0 = it.z * BATCH_RPW;
-	const int r1 = min(r0 + BATCH_RPW, I.rows);
-	if (I.cols % 2 == 0)
-		sweep_chunk<1, true>(z + I.off, h + I.off, hn + I.off, trash, I.rows, I.cols, strip, it.z, r0, r1, I.r, I.eps);
-	else
-		sweep_chunk<1, false>(z + I.off, h + I.off, hn + I.off, trash, I.rows, I.cols, strip, it.z, r0, r1, I.r,
-		                      I.eps);
+	const int L = wet_first_tile(ws, n_blocks, per_xcd);
+	if (L < 0) return;
+	const int k = __builtin_amdgcn_readfirstlane(L * FUSED_WAVES + (int)(threadIdx.x >> 6));
+	bool wet = false;
+	if (k < n_items) {
+		const int4 it = items[k];
+		const BatchInst I = inst[it.x];
+		const int strip = it.y, r0 = it.z * rpw;
+		const int r1 = min(r0 + rpw, I.rows);
+		if (I.cols % 2 == 0)
+			wet = sweep_chunk_al<D>(z + I.off, h + I.off, hn + I.off, trash, I.rows, I.cols, I.pitch, strip, it.z,
+			                        r0, r1, I.r, I.eps);
+		else   // odd column counts: the tile keeps its place in grid order
+			sweep_chunk<1, false>(z + I.off, h + I.off, hn + I.off, trash, I.rows, I.cols, strip, it.z, r0, r1,
+			                      I.r, I.eps);
+	}
+	wet_first_record(ws, L, wet);
 }
 
 struct DevBuf {
@@ -78,6 +94,9 @@ extern "C" int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols, 
 	}
 	std::vector<BatchInst> inst(n);
 	std::vector<int4> items;
+	int rpw = BATCH_RPW, depth = BATCH_DEPTH;
+	if (const char* e = getenv("CALHIP_BATCH_ROWS_PER_WAVE")) rpw = atoi(e) > 0 ? atoi(e) : rpw;
+	if (const char* e = getenv("CALHIP_BATCH_PREFETCH")) depth = atoi(e) > 0 ? atoi(e) : depth;
 	long long total = 0;
 	for (int k = 0; k < n; ++k) {
 		if (rows[k] < 3 || cols[k] < 3 || !z_host[k] || !h_host[k] || !h_out[k]) {
@@ -85,38 +104,43 @@ extern "C" int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols, 
 			return -1;
 		}
 		BatchInst& I = inst[k];
+		const bool even = cols[k] % 2 == 0;
 		I.rows = rows[k];
 		I.cols = cols[k];
-		I.strips = (cols[k] + STRIP2_OUT - 1) / STRIP2_OUT;
-		I.pad = 0;
+		I.strips = even ? (cols[k] + STRIP_AL - 1) / STRIP_AL : (cols[k] + STRIP2_OUT - 1) / STRIP2_OUT;
+		I.pitch = even ? (cols[k] + 15) / 16 * 16 : cols[k];
 		I.off = total;
 		I.r = p_r[k];
 		I.eps = p_eps[k];
-		total += ((long long)rows[k] * cols[k] + 31) / 32 * 32;   // 256-byte aligned starts
-		int sgroups = (I.strips + FUSED_WAVES - 1) / FUSED_WAVES;
-		int chunks = (rows[k] + BATCH_RPW - 1) / BATCH_RPW;
+		total += ((long long)rows[k] * I.pitch + 31) / 32 * 32;   // 256-byte aligned starts
+		const int chunks = (rows[k] + rpw - 1) / rpw;
 		for (int c = 0; c < chunks; ++c)
-			for (int g = 0; g < sgroups; ++g) items.push_back(make_int4(k, g, c, 0));
+			for (int st = 0; st < I.strips; ++st) items.push_back(make_int4(k, st, c, 0));
 	}
-	int n_items = (int)items.size();
-	int per_xcd = (n_items + 7) / 8;
+	const int n_items = (int)items.size();
+	const int n_blocks = (n_items + FUSED_WAVES - 1) / FUSED_WAVES;
+	const int per_xcd = (n_blocks + 7) / 8;
 
 	// host staging: z after the reference init (z -= h where h > 0), h
 	std::vector<double> hz((size_t)total, 0.0), hh((size_t)total, 0.0);
 	for (int k = 0; k < n; ++k) {
-		size_t cells = (size_t)rows[k] * cols[k];
-		double* zz = hz.data() + inst[k].off;
-		double* h0 = hh.data() + inst[k].off;
-		for (size_t c = 0; c < cells; ++c) {
-			double hv = h_host[k][c];
-			h0[c] = hv;
-			zz[c] = hv > 0.0 ? z_host[k][c] - hv : z_host[k][c];
+		for (int i = 0; i < rows[k]; ++i) {
+			const double* zs = z_host[k] + (size_t)i * cols[k];
+			const double* hs = h_host[k] + (size_t)i * cols[k];
+			double* zz = hz.data() + inst[k].off + (size_t)i * inst[k].pitch;
+			double* h0 = hh.data() + inst[k].off + (size_t)i * inst[k].pitch;
+			for (int j = 0; j < cols[k]; ++j) {
+				const double hv = hs[j];
+				h0[j] = hv;
+				zz[j] = hv > 0.0 ? zs[j] - hv : zs[j];
+			}
 		}
 	}
 
 	hipStream_t stream = nullptr;
 	if (CALHIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking))) return -1;
 	DevBuf dz, dh0, dh1, dinst, ditems, dtrash;
+	WetOrder wo;   // wet-first tile order (freed after the stream is synchronised)
 	size_t bytes = (size_t)total * sizeof(double);
 	int rc = -1;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -138,11 +162,24 @@ extern "C" int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols, 
 		if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
 		if (CALHIP_CHECK(hipEventRecord(e0, stream))) break;
 		double* bufs[2] = {(double*)dh0.p, (double*)dh1.p};
-		for (int s = 0; s < steps; ++s)
-			hipLaunchKernelGGL(batch_step, dim3(per_xcd * 8), dim3(64 * FUSED_WAVES), 0, stream,
-			                   (const BatchInst*)dinst.p, (const int4*)ditems.p, n_items, per_xcd,
-			                   (const double*)dz.p, (const double*)bufs[s & 1], bufs[(s + 1) & 1],
-			                   (double*)dtrash.p);
+		for (int s = 0; s < steps; ++s) {
+			WetSets ws = {};
+			int blocks = per_xcd * 8;
+			if (wo.next(stream, n_blocks, ws, 512) == 0)
+				blocks = 8 * (per_xcd + ws.cap);
+			else
+				ws = WetSets{};
+			if (depth >= 2)
+				hipLaunchKernelGGL((batch_step<2>), dim3(blocks), dim3(64 * FUSED_WAVES), 0, stream,
+				                   (const BatchInst*)dinst.p, (const int4*)ditems.p, n_items, n_blocks, per_xcd, rpw,
+				                   (const double*)dz.p, (const double*)bufs[s & 1], bufs[(s + 1) & 1],
+				                   (double*)dtrash.p, ws);
+			else
+				hipLaunchKernelGGL((batch_step<1>), dim3(blocks), dim3(64 * FUSED_WAVES), 0, stream,
+				                   (const BatchInst*)dinst.p, (const int4*)ditems.p, n_items, n_blocks, per_xcd, rpw,
+				                   (const double*)dz.p, (const double*)bufs[s & 1], bufs[(s + 1) & 1],
+				                   (double*)dtrash.p, ws);
+		}
 		if (CALHIP_CHECK(hipGetLastError())) break;
 		if (CALHIP_CHECK(hipEventRecord(e1, stream))) break;
 		if (CALHIP_CHECK(hipMemcpyAsync(hh.data(), bufs[steps & 1], bytes, hipMemcpyDeviceToHost, stream))) break;
@@ -151,7 +188,9 @@ extern "C" int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols, 
 		if (CALHIP_CHECK(hipEventElapsedTime(&ms, e0, e1))) break;
 		if (timing_ms) *timing_ms = ms;
 		for (int k = 0; k < n; ++k)
-			memcpy(h_out[k], hh.data() + inst[k].off, sizeof(double) * (size_t)rows[k] * cols[k]);
+			for (int i = 0; i < rows[k]; ++i)
+				memcpy(h_out[k] + (size_t)i * cols[k], hh.data() + inst[k].off + (size_t)i * inst[k].pitch,
+				       sizeof(double) * (size_t)cols[k]);
 		rc = 0;
 	} while (0);
 	if (e0) (void)hipEventDestroy(e0);

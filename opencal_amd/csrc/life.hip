@@ -7,14 +7,17 @@
 // bit-identical by construction.
 //
 // Kernels:
-//   life_step   one step, one thread per 4 consecutive cells of a row.
-//   life_multi  many steps of a grid that fits in one workgroup's LDS as a
-//               bit-packed board (two boards of up to 512 Ki cells = 128 KiB
-//               of the 160 KiB LDS; columns a multiple of 64): the board is
-//               loaded once, stepped with 64-cell bit-sliced adders, and
-//               written back once.  Used by calRun2D when no host callback
-//               runs between steps; the first step runs per-cell so the board
-//               holds only 0/1 values.
+//   life_step   one step, one thread per cell (grid-stride).
+//   life_tiles  k steps per launch by temporal blocking (columns a multiple of
+//               64): a block holds a bit-packed region of (tw + 2) 64-cell
+//               words x (tr + 2k) rows in LDS, advances it k steps with
+//               64-cell bit-sliced adders, and writes back its tile of tw
+//               words x tr rows.  A cell whose Moore neighbourhood leaves the
+//               region is wrong after one step and the error moves one cell
+//               per step, so after k <= 64 steps the tile (k rows and one word
+//               from every side of the region) is exact.  Used by calRun2D
+//               when no host callback runs between steps; the first step
+//               runs per cell so the board holds only 0/1 values.
 #include "engine.hpp"
 #include "device_common.hpp"
 
@@ -40,9 +43,11 @@ __global__ void life_step(const T* __restrict__ cur, T* __restrict__ nxt, int ro
 	}
 }
 
-// ---- LDS-resident bit-packed multi-step ----
-constexpr int LM_THREADS = 1024;
-constexpr int LM_MAX_WORDS = 16384;   // 64-bit words: 1 Mi cells, 128 KiB
+// ---- temporal blocking in LDS, bit-packed ----
+constexpr int LT_THREADS = 256;
+constexpr int LT_TW = 2;     // tile width in 64-cell words
+constexpr int LT_TR = 32;    // tile height in rows
+constexpr int LT_K = 32;     // steps per launch (<= 64: the side halo is one word)
 
 // bit-sliced full adder helpers on 64-cell words
 __device__ __forceinline__ void add2(uint64_t a, uint64_t b, uint64_t& s, uint64_t& c)
@@ -74,43 +79,100 @@ __device__ __forceinline__ uint64_t life_word(uint64_t nw, uint64_t n, uint64_t 
 	uint64_t t1, tc1, t2, tc2;
 	add3(c0, c1, c2, t1, tc1);
 	add2(t1, oc, t2, tc2);
-	// count = o + 2*t2 + 4*(tc1 + tc2 + ...) ; >= 4 iff tc1 | tc2
+	// count = o + 2*t2 + 4*(tc1 + tc2 + ...); >= 4 iff tc1 | tc2
 	uint64_t ge4 = tc1 | tc2;
 	uint64_t is3 = o & t2 & ~ge4;
 	uint64_t is2 = ~o & t2 & ~ge4;
 	return is3 | (is2 & c);
 }
 
-// Board layout in LDS: rows x wpr words, bit b of word (i, w) = cell (i, 64*w + b).
-// Toroidal wrap at the word level needs cols % 64 == 0 (checked on the host).
+__device__ __forceinline__ int wrapm(int x, int n)
+{
+	x %= n;
+	return x < 0 ? x + n : x;
+}
+
+// 64 cells of row i starting at column 64*q as a bit word (bit b = cell b).
 template <typename T>
-__global__ __launch_bounds__(LM_THREADS) void life_multi(T* __restrict__ cur, T* __restrict__ nxt, int rows,
-                                                         int cols, int steps)
+__device__ __forceinline__ uint64_t pack64(const T* __restrict__ row)
+{
+	uint64_t v = 0;
+	if (sizeof(T) == 1) {
+		const uint4* p = reinterpret_cast<const uint4*>(row);
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			const uint4 x = p[k];
+			const unsigned w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+			for (int d = 0; d < 4; ++d)
+#pragma unroll
+				for (int b = 0; b < 4; ++b)
+					v |= (uint64_t)(((w[d] >> (8 * b)) & 0xffu) == 1u) << (16 * k + 4 * d + b);
+		}
+	} else {
+		for (int b = 0; b < 64; ++b) v |= (uint64_t)(row[b] == 1) << b;
+	}
+	return v;
+}
+
+template <typename T>
+__device__ __forceinline__ void unpack64(uint64_t v, T* __restrict__ row)
+{
+	if (sizeof(T) == 1) {
+		uint4* p = reinterpret_cast<uint4*>(row);
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			unsigned w[4];
+#pragma unroll
+			for (int d = 0; d < 4; ++d) {
+				unsigned x = 0;
+#pragma unroll
+				for (int b = 0; b < 4; ++b) x |= (unsigned)((v >> (16 * k + 4 * d + b)) & 1u) << (8 * b);
+				w[d] = x;
+			}
+			p[k] = make_uint4(w[0], w[1], w[2], w[3]);
+		}
+	} else {
+		for (int b = 0; b < 64; ++b) row[b] = (T)((v >> b) & 1u);
+	}
+}
+
+// k steps of a rows x cols toroidal board (cols % 64 == 0), cur -> out.
+// Region word (ri, rq) holds board row i0 + ri and word column q0 + rq, both
+// wrapped (the region may be larger than the board: each region cell still
+// sees its true toroidal neighbours inside the region).
+template <typename T>
+__global__ __launch_bounds__(LT_THREADS) void life_tiles(const T* __restrict__ cur, T* __restrict__ out, int rows,
+                                                         int cols, int k)
 {
 	extern __shared__ uint64_t lds[];
 	const int wpr = cols / 64;
-	const int words = rows * wpr;
+	const int tiles_x = (wpr + LT_TW - 1) / LT_TW;
+	const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+	const int RW = LT_TW + 2, RH = LT_TR + 2 * k;
+	const int q0 = tx * LT_TW - 1, i0 = ty * LT_TR - k;
+	const int words = RW * RH;
 	uint64_t* A = lds;
 	uint64_t* B = lds + words;
-	for (int w = threadIdx.x; w < words; w += LM_THREADS) {
-		int i = w / wpr, q = w % wpr;
-		const T* src = cur + (size_t)i * cols + (size_t)q * 64;
-		uint64_t v = 0;
-		for (int b = 0; b < 64; ++b) v |= (uint64_t)(src[b] == 1) << b;
-		A[w] = v;
+	for (int w = threadIdx.x; w < words; w += LT_THREADS) {
+		const int ri = w / RW, rq = w % RW;
+		const int i = wrapm(i0 + ri, rows), q = wrapm(q0 + rq, wpr);
+		A[w] = pack64(cur + (size_t)i * cols + (size_t)q * 64);
 	}
 	__syncthreads();
-	for (int s = 0; s < steps; ++s) {
-		for (int w = threadIdx.x; w < words; w += LM_THREADS) {
-			int i = w / wpr, q = w % wpr;
-			int in = i == 0 ? rows - 1 : i - 1, is = i == rows - 1 ? 0 : i + 1;
-			int ql = q == 0 ? wpr - 1 : q - 1, qr = q == wpr - 1 ? 0 : q + 1;
-			uint64_t cn = A[in * wpr + q], cc = A[i * wpr + q], cs = A[is * wpr + q];
-			uint64_t ln = A[in * wpr + ql], lc = A[i * wpr + ql], ls = A[is * wpr + ql];
-			uint64_t rn = A[in * wpr + qr], rc = A[i * wpr + qr], rs = A[is * wpr + qr];
+	for (int s = 0; s < k; ++s) {
+		for (int w = threadIdx.x; w < words; w += LT_THREADS) {
+			const int ri = w / RW, rq = w % RW;
+			// neighbours outside the region read as dead
+			const bool hn = ri > 0, hs = ri < RH - 1, hl = rq > 0, hr = rq < RW - 1;
+			const uint64_t cn = hn ? A[w - RW] : 0, cc = A[w], cs = hs ? A[w + RW] : 0;
+			const uint64_t ln = (hn && hl) ? A[w - RW - 1] : 0, lc = hl ? A[w - 1] : 0;
+			const uint64_t ls = (hs && hl) ? A[w + RW - 1] : 0;
+			const uint64_t rn = (hn && hr) ? A[w - RW + 1] : 0, rc = hr ? A[w + 1] : 0;
+			const uint64_t rs = (hs && hr) ? A[w + RW + 1] : 0;
 			// west neighbour of bit b is bit b-1 (bit 63 of the left word for b = 0)
-			uint64_t wn = (cn << 1) | (ln >> 63), wc = (cc << 1) | (lc >> 63), ws = (cs << 1) | (ls >> 63);
-			uint64_t en = (cn >> 1) | (rn << 63), ec = (cc >> 1) | (rc << 63), es = (cs >> 1) | (rs << 63);
+			const uint64_t wn = (cn << 1) | (ln >> 63), wc = (cc << 1) | (lc >> 63), ws = (cs << 1) | (ls >> 63);
+			const uint64_t en = (cn >> 1) | (rn << 63), ec = (cc >> 1) | (rc << 63), es = (cs >> 1) | (rs << 63);
 			B[w] = life_word(wn, cn, en, wc, cc, ec, ws, cs, es);
 		}
 		__syncthreads();
@@ -118,13 +180,12 @@ __global__ __launch_bounds__(LM_THREADS) void life_multi(T* __restrict__ cur, T*
 		A = B;
 		B = t;
 	}
-	for (int w = threadIdx.x; w < words; w += LM_THREADS) {
-		int i = w / wpr, q = w % wpr;
-		T* dst = cur + (size_t)i * cols + (size_t)q * 64;
-		uint64_t v = A[w];
-		for (int b = 0; b < 64; ++b) dst[b] = (T)((v >> b) & 1u);
+	for (int w = threadIdx.x; w < LT_TW * LT_TR; w += LT_THREADS) {
+		const int ti = w / LT_TW, tq = w % LT_TW;
+		const int i = ty * LT_TR + ti, q = tx * LT_TW + tq;
+		if (i >= rows || q >= wpr) continue;
+		unpack64(A[(k + ti) * RW + 1 + tq], out + (size_t)i * cols + (size_t)q * 64);
 	}
-	(void)nxt;
 }
 
 }  // namespace
@@ -158,39 +219,38 @@ struct LifeProgram : Program {
 		return launch_step(m);
 	}
 
-	bool lds_fits(ModelX* m) const
-	{
-		return m->pub.columns % 64 == 0 && (size_t)m->pub.rows * (m->pub.columns / 64) * 2 <= (size_t)LM_MAX_WORDS;
-	}
+	bool tiles_fit(ModelX* m) const { return m->pub.columns % 64 == 0; }
 
 	int run_steps(ModelX* m, int n) override
 	{
 		if (q->nxt_dirty) flip(q);
-		if (n <= 1 || !lds_fits(m) || getenv("CALHIP_LIFE_NO_LDS")) {
+		if (n <= 1 || !tiles_fit(m) || getenv("CALHIP_LIFE_NO_TILES")) {
 			for (int k = 0; k < n; ++k)
 				if (launch_step(m)) return -1;
 			return 0;
 		}
 		if (launch_step(m)) return -1;   // afterwards every cell is 0 or 1
 		n -= 1;
-		size_t words = (size_t)m->pub.rows * (m->pub.columns / 64);
-		size_t lds = words * 2 * sizeof(uint64_t);
-		hipEvent_t ev;
-		timer_begin(m, &ev);
-		if (q->type == 'b') {
-			hipFuncSetAttribute((const void*)life_multi<int8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-			hipLaunchKernelGGL(life_multi<int8_t>, dim3(1), dim3(LM_THREADS), lds, m->stream, (int8_t*)q->d_cur,
-			                   (int8_t*)q->d_nxt, m->pub.rows, m->pub.columns, n);
-		} else {
-			hipFuncSetAttribute((const void*)life_multi<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-			hipLaunchKernelGGL(life_multi<int32_t>, dim3(1), dim3(LM_THREADS), lds, m->stream, (int32_t*)q->d_cur,
-			                   (int32_t*)q->d_nxt, m->pub.rows, m->pub.columns, n);
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		const int wpr = cols / 64;
+		const int blocks = ((wpr + LT_TW - 1) / LT_TW) * ((rows + LT_TR - 1) / LT_TR);
+		while (n > 0) {
+			const int k = n < LT_K ? n : LT_K;
+			const size_t lds = (size_t)2 * (LT_TW + 2) * (LT_TR + 2 * k) * sizeof(uint64_t);
+			hipEvent_t ev;
+			timer_begin(m, &ev);
+			if (q->type == 'b')
+				hipLaunchKernelGGL(life_tiles<int8_t>, dim3(blocks), dim3(LT_THREADS), lds, m->stream,
+				                   (const int8_t*)q->d_cur, (int8_t*)q->d_nxt, rows, cols, k);
+			else
+				hipLaunchKernelGGL(life_tiles<int32_t>, dim3(blocks), dim3(LT_THREADS), lds, m->stream,
+				                   (const int32_t*)q->d_cur, (int32_t*)q->d_nxt, rows, cols, k);
+			timer_end(m, ev);
+			m->stats.launches++;
+			if (CALHIP_CHECK(hipGetLastError())) return -1;
+			flip(q);
+			n -= k;
 		}
-		timer_end(m, ev);
-		m->stats.launches++;
-		if (CALHIP_CHECK(hipGetLastError())) return -1;
-		q->nxt_stale = true;   // the result is in place in the current layer
-		q->nxt_dirty = false;
 		return 0;
 	}
 };

@@ -181,7 +181,6 @@ struct SciddicaTProgram : Program {
 	const CALreal* p_r = nullptr;
 	const CALreal* p_eps = nullptr;
 	bool active = false;
-	int rpw = 0;   // row-chunk length of the fused sweep (0: not chosen yet)
 	bool h_prev_valid = false;   // h->d_nxt still holds the pre-step thickness
 
 	const char* name() const override { return "SciddicaT"; }
@@ -223,49 +222,27 @@ struct SciddicaTProgram : Program {
 		return 0;
 	}
 
-	// fused sweep launch: two columns per lane when the column count is even
-	int prefetch = 1;
 	double* d_trash = nullptr;   // sink for the stores of halo lanes
+	SweepShape shape = {0, 0};   // chunk length and prefetch depth (0: not chosen yet)
+	WetOrder wet_order;          // wet-first tile order of the fused sweep
 
 	~SciddicaTProgram() override
 	{
 		if (d_trash) hipFree(d_trash);
 	}
 
-	template <int D>
-	int launch2(ModelX* m, double r, double eps)
+	int launch_fused(ModelX* m, double r, double eps)
 	{
 		const int rows = m->pub.rows, cols = m->pub.columns;
 		if (!d_trash && CALHIP_CHECK(hipMalloc(&d_trash, 128 * sizeof(double)))) return -1;
-		// row-chunk length: autotuned per grid shape (the sweeps it times
-		// write into h's next layer, which this step overwrites)
-		if (rpw == 0)
-			rpw = sweep_rpw(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
-			                (double*)h->d_nxt, d_trash, r, eps);
-		if (D == 1) {
-			launch_sweep(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
-			             (double*)h->d_nxt, d_trash, r, eps, rpw);
-			return 0;
-		}
-		const SweepGeom g = sweep_geom(rows, cols, rpw);
-		if (cols % 2 == 0)
-			hipLaunchKernelGGL((step_fused2<D, true>), dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, m->stream,
-			                   (const double*)z->d_cur, (const double*)h->d_cur, (double*)h->d_nxt, d_trash, rows,
-			                   cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps);
-		else
-			hipLaunchKernelGGL((step_fused2<D, false>), dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, m->stream,
-			                   (const double*)z->d_cur, (const double*)h->d_cur, (double*)h->d_nxt, d_trash, rows,
-			                   cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps);
+		// sweep shape: autotuned per grid shape (the sweeps it times write
+		// into h's next layer, which this step overwrites)
+		if (shape.rpw == 0)
+			shape = sweep_shape(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
+			                    (double*)h->d_nxt, d_trash, r, eps);
+		launch_sweep(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur, (double*)h->d_nxt,
+		             d_trash, r, eps, shape, &wet_order);
 		return 0;
-	}
-
-	int launch_fused(ModelX* m, double r, double eps)
-	{
-		switch (prefetch) {
-		case 2: return launch2<2>(m, r, eps);
-		case 4: return launch2<4>(m, r, eps);
-		default: return launch2<1>(m, r, eps);
-		}
 	}
 
 	int check_entry(ModelX* m)
@@ -515,7 +492,6 @@ extern "C" int calhipBindSciddicaT2D(struct CALModel2D* ca2D, CALCallbackFunc2D 
 	p->p_r = p_r;
 	p->p_eps = p_epsilon;
 	p->active = ca2D->OPTIMIZATION == CAL_OPT_ACTIVE_CELLS;
-	if (const char* e = getenv("CALHIP_SCIDDICAT_PREFETCH")) p->prefetch = atoi(e);
 	delete m->prog;
 	m->prog = p;
 	m->prog_eps.clear();

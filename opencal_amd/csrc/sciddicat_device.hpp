@@ -274,6 +274,292 @@ __device__ __forceinline__ void sweep_chunk(const double* __restrict__ z, const 
 		sweep_strip<D, EVEN, false>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
 }
 
+// ---------------------------------------------------------------------------
+// Aligned strips (even column counts).  A wave owns the output columns
+// [128*strip, 128*strip + 128), two per lane, so every row access of the
+// strip is one aligned 1 KiB run and every store covers whole cache lines (a
+// strip that started mid-line would share partial lines with its neighbour
+// and cost about 12% of the sweep's bandwidth).  The two columns on either
+// side come with one more 16-byte load per row and array: lane 0 reads
+// columns j0-2, j0-1 and lane 1 reads jR, jR+1, where jR is the column after
+// the strip's last one (both wrap on the torus).  The outflows of the two
+// halo cells j0-1 (lane 0) and jR (lane 1) are a third minimisation chain;
+// the strip's first and last cells take their horizontal neighbour terms
+// from those lanes instead of from a lane shuffle.  `pitch` is the row
+// stride in elements (>= cols).
+
+struct TriFlows {
+	double a[4], b[4], c[4];
+};
+
+// pair_flows with a third cell c (computed only where c_on).
+__device__ __forceinline__ void tri_flows(double ha, double za, double ua1, double ua2, double ua3, double ua4,
+                                          double hb, double zb, double ub1, double ub2, double ub3, double ub4,
+                                          bool c_on, double hc, double zc, double uc1, double uc2, double uc3,
+                                          double uc4, double r, double eps, TriFlows& o, bool& seen)
+{
+#pragma unroll
+	for (int q = 0; q < 4; ++q) o.a[q] = o.b[q] = o.c[q] = 0.0;
+	const bool wa = !(ha <= eps), wb = !(hb <= eps), wc = c_on && !(hc <= eps);
+	if (!(wa || wb || wc)) return;
+	seen = true;
+	const double ma = ha - eps, mb = hb - eps, mc = hc - eps;
+	double u_a[5] = {za + eps, ua1, ua2, ua3, ua4};
+	double u_b[5] = {zb + eps, ub1, ub2, ub3, ub4};
+	double u_c[5] = {zc + eps, uc1, uc2, uc3, uc4};
+	unsigned ea = 0u, eb = 0u, ec = 0u;
+	double avg_a = 0.0, avg_b = 0.0, avg_c = 0.0;
+	bool ga = wa, gb = wb, gc = wc;
+	// as in pair_flows (chains a and b interleave); the halo chain c is wet
+	// in at most two lanes and its pass is skipped while no lane needs it
+	while (ga || gb || gc) {
+		bool na, nb;
+		avg_pass(u_a, ma, ea, avg_a, na);
+		avg_pass(u_b, mb, eb, avg_b, nb);
+		ga = ga && na;
+		gb = gb && nb;
+		if (__any((int)gc)) {
+			bool nc;
+			avg_pass(u_c, mc, ec, avg_c, nc);
+			gc = gc && nc;
+		}
+	}
+	if (wa) {
+#pragma unroll
+		for (int n = 1; n < 5; ++n) o.a[n - 1] = ((ea >> n) & 1u) ? 0.0 : (avg_a - u_a[n]) * r;
+	}
+	if (wb) {
+#pragma unroll
+		for (int n = 1; n < 5; ++n) o.b[n - 1] = ((eb >> n) & 1u) ? 0.0 : (avg_b - u_b[n]) * r;
+	}
+	if (wc) {
+#pragma unroll
+		for (int n = 1; n < 5; ++n) o.c[n - 1] = ((ec >> n) & 1u) ? 0.0 : (avg_c - u_c[n]) * r;
+	}
+}
+
+// Value of v in lane l (l wave-uniform).
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+	const long long b = __double_as_longlong(v);
+	const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+	const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+constexpr int STRIP_AL = 128;
+
+// Same stream of rows and the same arithmetic per cell as sweep_strip.
+// Returns (wave-uniform) whether any computed cell was wet.
+template <int D, bool UP>
+__device__ __forceinline__ bool sweep_strip_al(const double* __restrict__ z, const double* __restrict__ h,
+                                               double* __restrict__ hn, double* __restrict__ trash, int rows,
+                                               int cols, int pitch, int strip, int r0, int r1, double r, double eps)
+{
+	constexpr int W = D + 4;
+	const int lane = threadIdx.x & 63;
+	const int n_it = r1 - r0 + 2;   // flows computed for rows r0-1 .. r1
+	const int last_pos = n_it + 1;
+	const int j0 = strip * STRIP_AL;
+	const int ncol = min(STRIP_AL, cols - j0);   // even
+	const int kR = ncol / 2 - 1;                 // lane holding the strip's last column
+	const bool own = 2 * lane < ncol;
+	const int j = own ? j0 + 2 * lane : j0;      // lanes past the grid's edge read a valid pair (unused)
+	const int jR = j0 + ncol == cols ? 0 : j0 + ncol;
+	const int jL = j0 == 0 ? cols - 2 : j0 - 2;
+	const int jh = lane == 1 ? jR : jL;
+	const bool lane0 = lane == 0;
+	const bool c_on = lane < 2;
+
+	auto row_of = [&](int pos) -> int { return UP ? r1 + 1 - pos : r0 - 2 + pos; };
+	auto row_off = [&](int pos) -> size_t {
+		return (size_t)dev::wrap1(row_of(min(pos, last_pos)), rows) * (size_t)pitch;
+	};
+
+	// ring of rows: the lane's pair (bz, bh) and its halo pair (cz, ch)
+	double2 bz[W], bh[W], cz[W], ch[W];
+#pragma unroll
+	for (int q = 0; q < W - 1; ++q) {
+		const size_t o = row_off(q);
+		bz[q] = *reinterpret_cast<const double2*>(z + o + j);
+		bh[q] = *reinterpret_cast<const double2*>(h + o + j);
+		cz[q] = *reinterpret_cast<const double2*>(z + o + jh);
+		ch[q] = *reinterpret_cast<const double2*>(h + o + jh);
+		__builtin_amdgcn_sched_barrier(0);
+	}
+	bz[W - 1] = bh[W - 1] = cz[W - 1] = ch[W - 1] = make_double2(0.0, 0.0);
+
+	double pa0 = 0.0, pa1 = 0.0, pa2 = 0.0, pa3 = 0.0, pb0 = 0.0, pb1 = 0.0, pb2 = 0.0, pb3 = 0.0;
+	double pc1 = 0.0, pc2 = 0.0;   // W and E outflows of the halo cell of the row behind
+	double ppa = 0.0, ppb = 0.0;
+	bool seen = false;
+
+	for (int base = 0; base < n_it; base += W) {
+#pragma unroll
+		for (int d = 0; d < W; ++d) {
+			const int q = base + d;
+			const int sb = d % W, sc = (d + 1) % W, sa = (d + 2) % W, sp = (d + W - 1) % W;
+			{
+				const size_t o = row_off(q + W - 1);
+				bz[sp] = *reinterpret_cast<const double2*>(z + o + j);
+				bh[sp] = *reinterpret_cast<const double2*>(h + o + j);
+				cz[sp] = *reinterpret_cast<const double2*>(z + o + jh);
+				ch[sp] = *reinterpret_cast<const double2*>(h + o + jh);
+			}
+			const double2 zb = bz[sb], hb = bh[sb], zc = bz[sc], hc = bh[sc], za = bz[sa], ha = bh[sa];
+			const double2 zN = UP ? za : zb, hN = UP ? ha : hb, zS = UP ? zb : za, hS = UP ? hb : ha;
+			const double2 czc = cz[sc], chc = ch[sc];
+			const double2 czN = UP ? cz[sa] : cz[sb], chN = UP ? ch[sa] : ch[sb];
+			const double2 czS = UP ? cz[sb] : cz[sa], chS = UP ? ch[sb] : ch[sa];
+			const double ua = zc.x + hc.x, ub = zc.y + hc.y;
+			const double uhx = czc.x + chc.x, uhy = czc.y + chc.y;
+			const double ub_last = lane_value(ub, kR);   // level of the strip's last cell
+			const double u_right = lane_value(uhx, 1);   // level at column jR
+			double uw = dev::dpp_from_left(ub);
+			if (lane0) uw = uhy;
+			double ue = dev::dpp_from_right(ua);
+			if (lane == kR) ue = u_right;
+			// halo cell: column j0-1 in lane 0 (y of its pair), jR in lane 1 (x)
+			const double hc_c = lane0 ? chc.y : chc.x, zc_c = lane0 ? czc.y : czc.x;
+			const double uN_c = lane0 ? czN.y + chN.y : czN.x + chN.x;
+			const double uS_c = lane0 ? czS.y + chS.y : czS.x + chS.x;
+			const double uW_c = lane0 ? uhx : ub_last;
+			const double uE_c = lane0 ? ua : uhy;
+			TriFlows tf;
+			tri_flows(hc.x, zc.x, zN.x + hN.x, uw, ub, zS.x + hS.x, hc.y, zc.y, zN.y + hN.y, ua, ue, zS.y + hS.y,
+			          c_on, hc_c, zc_c, uN_c, uW_c, uE_c, uS_c, r, eps, tf, seen);
+			const double nA = UP ? tf.a[3] : ppa, sA = UP ? ppa : tf.a[0];
+			const double nB = UP ? tf.b[3] : ppb, sB = UP ? ppb : tf.b[0];
+			double f2w = dev::dpp_from_left(pb2);   // E outflow of a's west neighbour
+			if (lane0) f2w = pc2;
+			const double pc1_right = lane_value(pc1, 1);
+			double f1e = dev::dpp_from_right(pa1);  // W outflow of b's east neighbour
+			if (lane == kR) f1e = pc1_right;
+			double oa = hb.x, ob = hb.y;
+			oa += nA - pa0;
+			oa += f2w - pa1;
+			oa += pb1 - pa2;
+			oa += sA - pa3;
+			ob += nB - pb0;
+			ob += pa2 - pb1;
+			ob += f1e - pb2;
+			ob += sB - pb3;
+			const bool row_ok = q >= 2 && q < n_it;
+			double* dst = (row_ok && own) ? hn + (size_t)row_of(q) * (size_t)pitch + j : trash + 2 * lane;
+			*reinterpret_cast<double2*>(dst) = make_double2(oa, ob);
+			ppa = UP ? pa0 : pa3;
+			ppb = UP ? pb0 : pb3;
+			pa0 = tf.a[0];
+			pa1 = tf.a[1];
+			pa2 = tf.a[2];
+			pa3 = tf.a[3];
+			pb0 = tf.b[0];
+			pb1 = tf.b[1];
+			pb2 = tf.b[2];
+			pb3 = tf.b[3];
+			pc1 = tf.c[1];
+			pc2 = tf.c[2];
+			__builtin_amdgcn_sched_barrier(0);
+		}
+	}
+	return __any((int)seen) != 0;
+}
+
+template <int D>
+__device__ __forceinline__ bool sweep_chunk_al(const double* __restrict__ z, const double* __restrict__ h,
+                                               double* __restrict__ hn, double* __restrict__ trash, int rows,
+                                               int cols, int pitch, int strip, int chunk, int r0, int r1, double r,
+                                               double eps)
+{
+	if (chunk & 1) return sweep_strip_al<D, true>(z, h, hn, trash, rows, cols, pitch, strip, r0, r1, r, eps);
+	return sweep_strip_al<D, false>(z, h, hn, trash, rows, cols, pitch, strip, r0, r1, r, eps);
+}
+
+// Wet-first tile order.  A tile (FUSED_WAVES strips x one row chunk) that
+// met a wet cell costs several times a dry one: the minimisation runs for the
+// whole wave.  Dispatched in grid order, a wet tile that starts late ends the
+// launch late, so each launch records the tiles that met wet cells and the
+// next launch dispatches them first.  Per XCD group g (blockIdx & 7, which
+// owns the band of tiles [g*per_xcd, (g+1)*per_xcd)), block l < n runs wet
+// tile l of the group's list, and the following blocks walk the band in grid
+// order skipping the tiles of the list.  Every tile runs exactly once per
+// launch, and the order never changes a result.  The lists rotate over three
+// sets: a launch reads set e%3, appends to set (e+1)%3 and clears the lengths
+// of set (e+2)%3, which the next launch appends to.
+struct WetSets {
+	const unsigned* n_in;        // [8] list lengths (read)
+	const int* l_in;             // [8][cap] tiles (read)
+	unsigned* n_out;             // [8] lengths (appended)
+	int* l_out;                  // [8][cap]
+	unsigned* n_zero;            // [8] lengths cleared for the launch after
+	const unsigned char* f_in;   // [n_logical] 1 = tile is in l_in
+	unsigned char* f_out;        // [n_logical]
+	int cap;                     // list capacity per group
+};
+
+// The tile of this block under the wet-first order (ws.f_out set) or the
+// plain XCD-banded grid order; -1: the block has no tile.  Uniform per block.
+__device__ __forceinline__ int wet_first_tile(const WetSets& ws, int n_tiles, int per_xcd)
+{
+	const int g = blockIdx.x & 7, l = blockIdx.x >> 3;
+	if (!ws.f_out) {
+		const int L = dev::xcd_logical_block(blockIdx.x, per_xcd);
+		return L < n_tiles ? L : -1;
+	}
+	if (blockIdx.x == 0 && threadIdx.x < 8) ws.n_zero[threadIdx.x] = 0u;
+	const int nw = min((int)ws.n_in[g], ws.cap);
+	if (l < nw) return ws.l_in[g * ws.cap + l];
+	const int k = l - nw;
+	if (k >= per_xcd) return -1;
+	const int L = g * per_xcd + k;
+	if (L >= n_tiles || ws.f_in[L]) return -1;
+	return L;
+}
+
+// Records whether tile L met a wet cell (any wave of the block); every
+// thread of the block calls it.
+__device__ __forceinline__ void wet_first_record(const WetSets& ws, int L, bool wave_wet)
+{
+	if (!ws.f_out) return;
+	__shared__ int blk_wet;
+	if (threadIdx.x == 0) blk_wet = 0;
+	__syncthreads();
+	if (wave_wet && (threadIdx.x & 63) == 0) atomicOr(&blk_wet, 1);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		int w = blk_wet;
+		if (w) {
+			const int g = blockIdx.x & 7;
+			const unsigned j = atomicAdd(ws.n_out + g, 1u);
+			if ((int)j < ws.cap)
+				ws.l_out[g * ws.cap + j] = L;
+			else
+				w = 0;   // list full: the tile stays in grid order
+		}
+		ws.f_out[L] = (unsigned char)w;
+	}
+}
+
+// ws.f_out == nullptr: plain grid order (gridDim.x = 8 * per_xcd); otherwise
+// wet-first order (gridDim.x = 8 * (per_xcd + cap)).
+template <int D>
+__global__ __launch_bounds__(64 * FUSED_WAVES) void step_fused_al(
+    const double* __restrict__ z, const double* __restrict__ h, double* __restrict__ hn, double* __restrict__ trash,
+    int rows, int cols, int n_strips, int n_logical, int per_xcd, int rpw, double r, double eps, WetSets ws)
+{
+	const int L = wet_first_tile(ws, n_logical, per_xcd);
+	if (L < 0) return;
+	// wave items in row-major (chunk, strip) order; a block is FUSED_WAVES
+	// consecutive items (the item index is wave-uniform: a scalar register)
+	const int item = __builtin_amdgcn_readfirstlane(L * FUSED_WAVES + (int)(threadIdx.x >> 6));
+	const int strip = item % n_strips, chunk = item / n_strips;
+	const int r0 = chunk * rpw;
+	bool wet = false;
+	if (r0 < rows)
+		wet = sweep_chunk_al<D>(z, h, hn, trash, rows, cols, cols, strip, chunk, r0, min(r0 + rpw, rows), r, eps);
+	wet_first_record(ws, L, wet);
+}
+
 template <int D, bool EVEN>
 __global__ __launch_bounds__(64 * FUSED_WAVES) void step_fused2(
     const double* __restrict__ z, const double* __restrict__ h, double* __restrict__ hn, double* __restrict__ trash,
@@ -298,14 +584,29 @@ struct SweepGeom {
 	int n_strips, n_logical, per_xcd, blocks, rpw;
 };
 SweepGeom sweep_geom(int rows, int cols, int rpw);
-// One fused step h -> hn (prefetch depth 1) on stream st.
+// Shape of the sweep: row-chunk length and prefetch depth.
+struct SweepShape {
+	int rpw, depth;
+};
+// Device state of the wet-first order for one grid shape (owned by a model,
+// an adjoint run, ...); allocated on first use, rebuilt when the shape changes.
+struct WetOrder {
+	void* buf = nullptr;
+	int n_logical = 0, cap = 0;
+	unsigned long long epoch = 0;
+	~WetOrder();
+	// the sets of the next launch (advances the epoch); -1 on allocation failure
+	int next(hipStream_t st, int n_logical, WetSets& ws, int max_cap = 128);
+};
+// One fused step h -> hn on stream st (wo: wet-first order, or nullptr).
 void launch_sweep(hipStream_t st, int rows, int cols, const double* z, const double* h, double* hn, double* trash,
-                  double r, double eps, int rpw);
-// The row-chunk length for this grid shape: the environment's
-// CALHIP_SCIDDICAT_ROWS_PER_WAVE, else the autotuned one (timed on first use
-// by writing sweeps of h into `scratch`, then cached per shape).
-int sweep_rpw(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch, double* trash,
-              double r, double eps);
+                  double r, double eps, SweepShape shape, WetOrder* wo);
+// The sweep shape for this grid: the environment's
+// CALHIP_SCIDDICAT_ROWS_PER_WAVE / CALHIP_SCIDDICAT_PREFETCH, else the
+// autotuned one (timed on first use by writing sweeps of h into `scratch`,
+// then cached per grid shape).
+SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch,
+                       double* trash, double r, double eps);
 
 }  // namespace sct
 }  // namespace calhip

@@ -1,11 +1,13 @@
-// Launch shape of the fused SciddicaT sweep and its autotuner.
+// Launch shape of the fused SciddicaT sweep, its wet-first tile order and its
+// autotuner.
 //
-// The sweep (step_fused2) gives every wave one 124-column strip of a chunk of
-// `rpw` rows.  Short chunks make more waves (needed to fill 256 CUs on small
-// grids, and they bound the tail of the few waves that meet wet cells) but
-// re-read more halo rows; long chunks stream better on large grids.  The
-// autotuner times each candidate chunk length on the caller's stream with HIP
-// events (a few launches each, writing into a buffer the caller is about to
+// The sweep (step_fused_al for even column counts, step_fused2 for odd ones)
+// gives every wave one 128-column (124 for odd counts) strip of a chunk of
+// `rpw` rows, prefetching `depth` rows ahead.  Short chunks make more waves
+// and bound the time of a wave that meets wet cells, but re-read more halo
+// rows; long chunks and deeper prefetch stream better on large grids.  The
+// autotuner times each candidate shape on the caller's stream with HIP events
+// (a few launches each, writing into a buffer the caller is about to
 // overwrite anyway) and keeps the fastest per grid shape for the life of the
 // process.  Every candidate computes the same bits, so the choice never
 // changes a result.
@@ -26,64 +28,142 @@ SweepGeom sweep_geom(int rows, int cols, int rpw)
 {
 	SweepGeom g;
 	g.rpw = rpw;
-	g.n_strips = (cols + STRIP2_OUT - 1) / STRIP2_OUT;
-	const int sgroups = (g.n_strips + FUSED_WAVES - 1) / FUSED_WAVES;
-	g.n_logical = sgroups * ((rows + rpw - 1) / rpw);
+	// even column counts: aligned 128-column strips; odd: 124-column strips
+	g.n_strips = cols % 2 == 0 ? (cols + STRIP_AL - 1) / STRIP_AL : (cols + STRIP2_OUT - 1) / STRIP2_OUT;
+	if (cols % 2 == 0) {
+		// aligned sweep: blocks of FUSED_WAVES consecutive (chunk, strip) items
+		g.n_logical = (g.n_strips * ((rows + rpw - 1) / rpw) + FUSED_WAVES - 1) / FUSED_WAVES;
+	} else {
+		// 124-column sweep: blocks of FUSED_WAVES strips of one chunk
+		const int sgroups = (g.n_strips + FUSED_WAVES - 1) / FUSED_WAVES;
+		g.n_logical = sgroups * ((rows + rpw - 1) / rpw);
+	}
 	g.per_xcd = (g.n_logical + 7) / 8;
 	g.blocks = g.per_xcd * 8;
 	return g;
 }
 
-void launch_sweep(hipStream_t st, int rows, int cols, const double* z, const double* h, double* hn, double* trash,
-                  double r, double eps, int rpw)
+// ------------------------------------------------------------- wet order
+
+namespace {
+constexpr size_t WO_COUNTS = 3 * 8 * sizeof(unsigned);
+size_t wo_lists_off() { return 256; }
+size_t wo_flags_off(int cap) { return wo_lists_off() + (((size_t)3 * 8 * cap * sizeof(int) + 255) & ~(size_t)255); }
+}  // namespace
+
+WetOrder::~WetOrder()
 {
-	const SweepGeom g = sweep_geom(rows, cols, rpw);
-	if (cols % 2 == 0)
-		hipLaunchKernelGGL((step_fused2<1, true>), dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, st, z, h, hn, trash,
-		                   rows, cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps);
-	else
+	if (buf) (void)hipFree(buf);
+}
+
+int WetOrder::next(hipStream_t st, int nl, WetSets& ws, int max_cap)
+{
+	int want_cap = (nl + 7) / 8;
+	if (want_cap > max_cap) want_cap = max_cap;
+	if (!buf || nl != n_logical || want_cap != cap) {
+		if (buf) (void)hipFree(buf);
+		buf = nullptr;
+		n_logical = nl;
+		// wet tiles beyond the capacity run in grid order; each unused slot
+		// costs one block that exits at once
+		cap = want_cap;
+		const size_t bytes = wo_flags_off(cap) + 2 * (size_t)nl;
+		static_assert(WO_COUNTS <= 256, "counts fit the first 256 bytes");
+		if (CALHIP_CHECK(hipMalloc(&buf, bytes))) {
+			buf = nullptr;
+			return -1;
+		}
+		if (CALHIP_CHECK(hipMemsetAsync(buf, 0, bytes, st))) return -1;
+		epoch = 0;
+	}
+	char* b = (char*)buf;
+	unsigned* counts = (unsigned*)b;
+	int* lists = (int*)(b + wo_lists_off());
+	unsigned char* flags = (unsigned char*)(b + wo_flags_off(cap));
+	const int si = (int)(epoch % 3), so = (int)((epoch + 1) % 3), sz = (int)((epoch + 2) % 3);
+	ws.n_in = counts + 8 * si;
+	ws.l_in = lists + (size_t)8 * cap * si;
+	ws.n_out = counts + 8 * so;
+	ws.l_out = lists + (size_t)8 * cap * so;
+	ws.n_zero = counts + 8 * sz;
+	ws.f_in = flags + (size_t)nl * (epoch & 1);
+	ws.f_out = flags + (size_t)nl * ((epoch + 1) & 1);
+	ws.cap = cap;
+	epoch++;
+	return 0;
+}
+
+void launch_sweep(hipStream_t st, int rows, int cols, const double* z, const double* h, double* hn, double* trash,
+                  double r, double eps, SweepShape shape, WetOrder* wo)
+{
+	const SweepGeom g = sweep_geom(rows, cols, shape.rpw);
+	if (cols % 2 != 0) {
 		hipLaunchKernelGGL((step_fused2<1, false>), dim3(g.blocks), dim3(64 * FUSED_WAVES), 0, st, z, h, hn, trash,
 		                   rows, cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps);
+		return;
+	}
+	WetSets ws = {};
+	int blocks = g.blocks;
+	if (wo && wo->next(st, g.n_logical, ws) == 0)
+		blocks = 8 * (g.per_xcd + ws.cap);
+	else
+		ws = WetSets{};
+	if (shape.depth >= 2)
+		hipLaunchKernelGGL((step_fused_al<2>), dim3(blocks), dim3(64 * FUSED_WAVES), 0, st, z, h, hn, trash, rows,
+		                   cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps, ws);
+	else
+		hipLaunchKernelGGL((step_fused_al<1>), dim3(blocks), dim3(64 * FUSED_WAVES), 0, st, z, h, hn, trash, rows,
+		                   cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps, ws);
 }
+
+// --------------------------------------------------------------- tuner
 
 namespace {
 std::mutex tune_mu;
-std::map<std::pair<int, int>, int> tuned;
+std::map<std::pair<int, int>, SweepShape> tuned;
 }  // namespace
 
-int sweep_rpw(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch, double* trash,
-              double r, double eps)
+SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch,
+                       double* trash, double r, double eps)
 {
-	if (const char* e = getenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE")) {
-		const int v = atoi(e);
-		if (v > 0) return v;
-	}
+	SweepShape fixed = {0, 0};
+	if (const char* e = getenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE")) fixed.rpw = atoi(e);
+	if (const char* e = getenv("CALHIP_SCIDDICAT_PREFETCH")) fixed.depth = atoi(e);
+	if (fixed.rpw > 0 && fixed.depth > 0) return fixed;
 	std::lock_guard<std::mutex> lk(tune_mu);
 	const auto key = std::make_pair(rows, cols);
 	auto it = tuned.find(key);
-	if (it != tuned.end()) return it->second;
-	static const int cand[] = {2, 4, 8};
-	hipEvent_t e0 = nullptr, e1 = nullptr;
-	int best = 8;
-	float best_ms = 0.f;
-	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-		if (e0) (void)hipEventDestroy(e0);
-		return best;   // no timing: the default shape
+	if (it != tuned.end()) {
+		SweepShape s = it->second;
+		if (fixed.rpw > 0) s.rpw = fixed.rpw;
+		if (fixed.depth > 0) s.depth = fixed.depth;
+		return s;
 	}
-	for (int rpw : cand) {
-		launch_sweep(st, rows, cols, z, h, scratch, trash, r, eps, rpw);   // warm
-		(void)hipEventRecord(e0, st);
-		for (int k = 0; k < 3; ++k) launch_sweep(st, rows, cols, z, h, scratch, trash, r, eps, rpw);
-		(void)hipEventRecord(e1, st);
-		float ms = 0.f;
-		if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
-		if (best_ms == 0.f || ms < best_ms) {
-			best_ms = ms;
-			best = rpw;
+	static const SweepShape cand[] = {{8, 1}, {12, 1}, {16, 1}, {24, 1}, {32, 1}, {8, 2}, {16, 2}, {24, 2}, {32, 2}};
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	SweepShape best = {16, 1};
+	float best_ms = 0.f;
+	if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+		for (SweepShape c : cand) {
+			if (fixed.rpw > 0 && c.rpw != fixed.rpw) continue;
+			if (fixed.depth > 0 && c.depth != fixed.depth) continue;
+			if (cols % 2 != 0 && c.depth != 1) continue;   // odd counts have one kernel depth
+			WetOrder wo;   // its own order state: the first launch learns the wet tiles
+			launch_sweep(st, rows, cols, z, h, scratch, trash, r, eps, c, &wo);
+			(void)hipEventRecord(e0, st);
+			for (int k = 0; k < 3; ++k) launch_sweep(st, rows, cols, z, h, scratch, trash, r, eps, c, &wo);
+			(void)hipEventRecord(e1, st);
+			float ms = 0.f;
+			if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+			if (best_ms == 0.f || ms < best_ms) {
+				best_ms = ms;
+				best = c;
+			}
 		}
 	}
-	(void)hipEventDestroy(e0);
-	(void)hipEventDestroy(e1);
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	(void)hipGetLastError();
 	tuned[key] = best;
 	return best;
 }

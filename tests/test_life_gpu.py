@@ -28,6 +28,18 @@ def test_life_vs_oracle(rows, cols, steps):
     assert int(got.sum()) == int(want.sum())
 
 
+@pytest.mark.parametrize("rows,cols,steps", [(64, 64, 200), (8, 192, 100), (100, 320, 66), (33, 128, 2),
+                                             (70, 256, 33), (3, 64, 40)])
+def test_life_tiles_edges(rows, cols, steps):
+    """calRun2D's temporally blocked path: boards narrower than a tile or
+    shorter than a region (the region wraps onto itself), step counts that
+    leave a partial last launch, and a single multi-step step."""
+    from opencal_amd.models import life_run
+    print(f"case life tiles {rows}x{cols} steps={steps}", flush=True)
+    a0 = synth.life_inputs(rows, cols, seed=rows * 11 + cols)
+    assert np.array_equal(life_run(a0, steps), np_life(a0, steps))
+
+
 def test_life_stepwise_equals_batched():
     from opencal_amd.models import Life
     a0 = synth.life_inputs(128, 128, seed=5)

@@ -38,6 +38,27 @@ def test_bit_exact_vs_oracle(rows, cols, steps, seed, active):
         assert gcount == ocount[-1]
 
 
+@pytest.mark.parametrize("rows,cols", [(7, 4), (33, 130), (40, 260), (21, 254), (50, 384), (19, 132)])
+def test_strip_edges(rows, cols):
+    """Even column counts run in 128-column strips whose first and last
+    cells take their horizontal terms from separately loaded halo columns:
+    wet cells straddle every strip boundary and the toroidal wrap, and the
+    last strip holds 2, 4, 126 or 128 columns."""
+    print(f"case strip edges rows={rows} cols={cols}", flush=True)
+    z, h, _ = synth.sciddicat_inputs(rows, cols, 31, patch=1)
+    h[:] = 0.0
+    s = synth.Stream(rows * 1000 + cols)
+    for j in list(range(0, cols, 128)) + [cols - 1]:
+        for dj in (-3, -2, -1, 0, 1, 2):
+            jj = (j + dj) % cols
+            h[:, jj] = s.between(0.5, 4.0, rows)
+    h[0, :] = s.between(0.0, 2.0, cols)
+    for active in (False, True):
+        gz, gh, _ = gpu_run(z, h, 0.5, 0.001, 60, active)
+        oz, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 60, active)
+        assert np.array_equal(gh, oh), f"active={active} max |dh| = {np.abs(gh - oh).max()}"
+
+
 @pytest.mark.parametrize("p_r,p_eps", [(0.3, 1e-4), (0.7, 1e-2), (0.61, 0.0031)])
 def test_parameters(p_r, p_eps):
     z, h, _ = synth.sciddicat_inputs(80, 72, 9, patch=12)
