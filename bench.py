@@ -38,6 +38,20 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_SCIDDICAT = 24.0   # z read, h read, h written (FP64)
+SWEEP_PROFILE = os.path.join(ROOT, "profiles", "sciddicat_sweep_4096x4096.md")
+
+
+def profiled_traffic(path=SWEEP_PROFILE):
+    """HBM bytes per launch of the sweep from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated against a device copy);
+    None when the summary is absent."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -333,8 +347,10 @@ def main():
                    "cols": args.size, "p_r": 0.5, "p_epsilon": 0.001, "optimization": "CAL_NO_OPT",
                    "driver": "calRunCAStep2D per step (C API)"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "step_fused2", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
+                     "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic() if args.size == 4096 else None,
+                     "traffic_source": "profiles/sciddicat_sweep_4096x4096.md (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE)",
+                     "kernel": "step_fused_al", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
                      "bytes_per_cell": BYTES_PER_CELL_SCIDDICAT},
     }
     log("pcie-inclusive run ...")

@@ -29,7 +29,7 @@ namespace {
 using namespace sct;
 
 constexpr int BATCH_RPW = 16;   // rows per chunk
-constexpr int BATCH_DEPTH = 2;  // rows prefetched ahead by the aligned sweep
+constexpr int BATCH_DEPTH = 1;  // rows prefetched ahead by the aligned sweep
 
 struct BatchInst {
 	int rows, cols, strips, pitch;   // pitch: row stride in elements
