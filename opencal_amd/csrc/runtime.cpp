@@ -197,6 +197,7 @@ int device_fill(ModelX* m, SubX* s, double v, bool both)
 	}
 	if (m->prog && m->prog->before_modify(m, s)) return -1;
 	bool partial = m->active();   // over the active list when it is non-empty
+	if (partial && m->prog && m->prog->sync_active(m)) return -1;
 	if (s->single) return k::fill(m, s->d_cur, s->elem, v, partial);
 	if (both) {
 		if (k::fill(m, s->d_cur, s->elem, v, partial)) return -1;

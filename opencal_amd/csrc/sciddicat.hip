@@ -35,6 +35,7 @@
 // to the CPU path.
 #include <stdio.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "engine.hpp"
@@ -164,6 +165,96 @@ __global__ void zero_at_active(FlowPtrs f, const int* __restrict__ cells, const 
 	}
 }
 
+// The active step driven by the flag grid instead of the row-major list.
+// A flag byte holds bit 0 = in the set at the start of the step, bit 1 =
+// joined the set as a receiver in this step's flows process.  The set sizes
+// are device counters, so neither launch depends on a host value and no list
+// is rebuilt between the processes; the list is rebuilt (row-major) only when
+// it is observed.  Every cell processed is one the list path would process:
+// flows visits the set (every cell while it is empty, as OpenCAL does when
+// A.cells == NULL), width and removal visit the set plus its receivers.
+
+// flows over the set; receivers get bit 1 through a 32-bit atomic OR on the
+// word that holds their flag (the flag grid is padded to whole 16-byte
+// words), and first additions are counted into *cnt_added.  Clears the two
+// counters the next step accumulates into.
+__global__ __launch_bounds__(256) void flows_flags(const double* __restrict__ z, const double* __restrict__ h,
+                                                   FlowPtrs f, uint8_t* __restrict__ flags,
+                                                   const int* __restrict__ cnt_cur, int* __restrict__ cnt_added,
+                                                   int* __restrict__ clear_a, int* __restrict__ clear_b, int rows,
+                                                   int cols, double r, double eps, int zero_dry)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		*clear_a = 0;
+		*clear_b = 0;
+	}
+	const size_t n = (size_t)rows * cols;
+	const bool all = cnt_cur[0] == 0;
+	int added = 0;
+	for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+		if (!all && !(flags[c] & 1u)) continue;
+		const int i = (int)(c / cols), j = (int)(c % cols);
+		const double h0 = h[c];
+		if (h0 <= eps) {
+			if (zero_dry)
+				for (int q = 0; q < 4; ++q) f.f[q][c] = 0.0;
+			continue;
+		}
+		size_t nb[4];
+		nb[0] = (size_t)dev::wrap1(i - 1, rows) * cols + j;
+		nb[1] = (size_t)i * cols + dev::wrap1(j - 1, cols);
+		nb[2] = (size_t)i * cols + dev::wrap1(j + 1, cols);
+		nb[3] = (size_t)dev::wrap1(i + 1, rows) * cols + j;
+		double fl[4];
+		const unsigned keep = cell_flows(h0, z[c], z[nb[0]] + h[nb[0]], z[nb[1]] + h[nb[1]], z[nb[2]] + h[nb[2]],
+		                                 z[nb[3]] + h[nb[3]], r, eps, fl);
+		for (int q = 0; q < 4; ++q) {
+			f.f[q][c] = fl[q];
+			if ((keep >> (q + 1)) & 1u) {
+				const unsigned sh = (unsigned)(nb[q] & 3u) * 8u;
+				const unsigned old = atomicOr(reinterpret_cast<unsigned*>(flags) + (nb[q] >> 2), 2u << sh);
+				if (((old >> sh) & 0xffu) == 0u) added++;
+			}
+		}
+	}
+	for (int off = 32; off > 0; off >>= 1) added += __shfl_down(added, off, 64);
+	if ((threadIdx.x & 63) == 0 && added) atomicAdd(cnt_added, added);
+}
+
+// width over the set and its receivers (every cell while both are empty),
+// in place, fused with the removal process: a visited cell stays in the set
+// (flag 1) when it was in it and not h' <= eps; its receiver bit is cleared.
+// The size of the new set is counted into *cnt_next.
+__global__ __launch_bounds__(256) void width_flags(double* __restrict__ h, FlowPtrs f, uint8_t* __restrict__ flags,
+                                                   const int* __restrict__ cnt_cur,
+                                                   const int* __restrict__ cnt_added, int* __restrict__ cnt_next,
+                                                   int rows, int cols, double eps)
+{
+	const size_t n = (size_t)rows * cols;
+	const bool all = cnt_cur[0] + cnt_added[0] == 0;
+	int kept = 0;
+	for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+		const unsigned fl = flags[c];
+		if (!all && !(fl & 3u)) continue;
+		const int i = (int)(c / cols), j = (int)(c % cols);
+		const size_t cn = (size_t)dev::wrap1(i - 1, rows) * cols + j;
+		const size_t cw = (size_t)i * cols + dev::wrap1(j - 1, cols);
+		const size_t ce = (size_t)i * cols + dev::wrap1(j + 1, cols);
+		const size_t cs = (size_t)dev::wrap1(i + 1, rows) * cols + j;
+		double o = h[c];
+		o += f.f[3][cn] - f.f[0][c];
+		o += f.f[2][cw] - f.f[1][c];
+		o += f.f[1][ce] - f.f[2][c];
+		o += f.f[0][cs] - f.f[3][c];
+		h[c] = o;
+		const unsigned nf = ((fl & 3u) && !(o <= eps)) ? 1u : 0u;
+		if (nf != fl) flags[c] = (uint8_t)nf;
+		kept += (int)nf;
+	}
+	for (int off = 32; off > 0; off >>= 1) kept += __shfl_down(kept, off, 64);
+	if ((threadIdx.x & 63) == 0 && kept) atomicAdd(cnt_next, kept);
+}
+
 int stride_grid(size_t n, int block)
 {
 	size_t b = (n + block - 1) / block;
@@ -219,6 +310,8 @@ struct SciddicaTProgram : Program {
 		}
 		set_flow_state(zero ? (active ? Transient::ZeroAtActive : Transient::Zero) : Transient::None);
 		h_prev_valid = false;
+		ctr_valid = false;    // the upload brought the list and its count
+		list_stale = false;
 		return 0;
 	}
 
@@ -226,9 +319,26 @@ struct SciddicaTProgram : Program {
 	SweepShape shape = {0, 0};   // chunk length and prefetch depth (0: not chosen yet)
 	WetOrder wet_order;          // wet-first tile order of the fused sweep
 
+	// flag-driven active step: set sizes [0..1] (current / next by parity)
+	// and receiver counts [2..3]; the row-major list is rebuilt on demand
+	int* d_ctr = nullptr;
+	int parity = 0;
+	bool ctr_valid = false;      // d_ctr[parity] holds the size of the set
+	bool list_stale = false;     // m->d_cells / d_count lag behind the flags
+
 	~SciddicaTProgram() override
 	{
 		if (d_trash) hipFree(d_trash);
+		if (d_ctr) hipFree(d_ctr);
+	}
+
+	int sync_active(ModelX* m) override
+	{
+		if (!list_stale) return 0;
+		if (k::rebuild_active(m)) return -1;
+		m->stats.rebuilds++;
+		list_stale = false;
+		return 0;
 	}
 
 	int launch_fused(ModelX* m, double r, double eps)
@@ -311,6 +421,58 @@ struct SciddicaTProgram : Program {
 	// ---- active cells ----
 	int step_active(ModelX* m)
 	{
+		// the host changed the flags since the last update: the list (not the
+		// flags) is what this step's first process visits
+		if (m->flags_dirty) {
+			ctr_valid = false;
+			return step_active_list(m);
+		}
+		const double r = *p_r, eps = *p_eps;
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		const size_t n = m->cells();
+		if (!d_ctr) {
+			if (CALHIP_CHECK(hipMalloc(&d_ctr, 4 * sizeof(int)))) return -1;
+			if (CALHIP_CHECK(hipMemsetAsync(d_ctr, 0, 4 * sizeof(int), m->stream))) return -1;
+			ctr_valid = false;
+		}
+		if (!ctr_valid) {
+			// the list is current: its length is the size of the set
+			if (sync_active(m)) return -1;
+			if (CALHIP_CHECK(hipMemcpyAsync(d_ctr + parity, m->d_count, sizeof(int), hipMemcpyDeviceToDevice,
+			                                m->stream)) ||
+			    CALHIP_CHECK(hipMemsetAsync(d_ctr + 2 + parity, 0, sizeof(int), m->stream)))
+				return -1;
+			ctr_valid = true;
+		}
+		int zero_dry = 1;
+		for (int q = 0; q < 4; ++q)
+			if (f[q]->tstate != Transient::ZeroAtActive && f[q]->tstate != Transient::Zero) zero_dry = 0;
+		int* cur = d_ctr + parity;
+		int* nxt = d_ctr + (1 - parity);
+		int* added = d_ctr + 2 + parity;
+		int* added_next = d_ctr + 2 + (1 - parity);
+		const int grid = (int)std::min<size_t>((n + 255) / 256, 65535);
+		hipEvent_t ev;
+		timer_begin(m, &ev);
+		hipLaunchKernelGGL(flows_flags, dim3(grid), dim3(256), 0, m->stream, (const double*)z->d_cur,
+		                   (const double*)h->d_cur, cur_flows(), m->d_flags, (const int*)cur, added, nxt,
+		                   added_next, rows, cols, r, eps, zero_dry);
+		hipLaunchKernelGGL(width_flags, dim3(grid), dim3(256), 0, m->stream, (double*)h->d_cur, cur_flows(),
+		                   m->d_flags, (const int*)cur, (const int*)added, nxt, rows, cols, eps);
+		timer_end(m, ev);
+		m->stats.launches += 2;
+		if (CALHIP_CHECK(hipGetLastError())) return -1;
+		parity = 1 - parity;
+		list_stale = true;
+		m->flags_dirty = false;
+		set_flow_state(Transient::DerivedActive);
+		return 0;
+	}
+
+	// The same step over the row-major list, rebuilt after each process.
+	int step_active_list(ModelX* m)
+	{
+		if (sync_active(m)) return -1;
 		const double r = *p_r, eps = *p_eps;
 		const int rows = m->pub.rows, cols = m->pub.columns;
 		size_t n = m->cells();
@@ -435,6 +597,7 @@ struct SciddicaTProgram : Program {
 			}
 			FlowPtrs tgt;
 			for (int q = 0; q < 4; ++q) tgt.f[q] = (double*)(zero_cur ? f[q]->d_cur : f[q]->d_nxt);
+			if (zero_nxt && sync_active(m)) return -1;
 			if (zero_nxt) {
 				hipLaunchKernelGGL(zero_at_active, dim3(grid), dim3(256), 0, m->stream, tgt, m->d_cells, m->d_count, n);
 				m->stats.launches++;

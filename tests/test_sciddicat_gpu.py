@@ -111,6 +111,28 @@ def test_flows_observable_after_step():
     m.close()
 
 
+@pytest.mark.parametrize("split", [1, 7, 30])
+def test_active_round_trips(split):
+    """Active cells stepped on the flag grid: the row-major list is rebuilt
+    only when observed.  Reading the state (list, count, substates) in the
+    middle of the run and continuing gives the oracle's result and counts."""
+    from opencal_amd.models import SciddicaT
+    z, h, _ = synth.sciddicat_inputs(90, 128, 44, patch=10)
+    m = SciddicaT(z, h, 0.5, 0.001, 60, True)
+    m.init()
+    _, oh_all, counts = native.oracle_sciddicat(z, h, 0.5, 0.001, 60, True, counts=True)
+    m.steps(split)
+    _, gh, cnt = m.read()
+    _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, split, True)
+    assert np.array_equal(gh, oh)
+    assert cnt == counts[split - 1]
+    m.steps(60 - split)
+    _, gh, cnt = m.read()
+    assert np.array_equal(gh, oh_all)
+    assert cnt == counts[-1]
+    m.close()
+
+
 def test_mass_conservation_4096():
     """At the benchmark size: total thickness is conserved on the torus (to
     rounding), and the first 3 steps match the oracle bit for bit."""
