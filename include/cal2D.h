@@ -81,8 +81,10 @@ struct CALModel2D* calCADef2D(int rows, int columns, enum CALNeighborhood2D CAL_
                               enum CALSpaceBoundaryCondition CAL_TOROIDALITY,
                               enum CALOptimization CAL_OPTIMIZATION);
 
-/* cal2D.c:329 — accepted for API compatibility; unsafe neighbour writes are
- * not on the device path (calling calSetX2D* is not provided). */
+/* cal2D.c:329 — marks the model as using the unsafe operations
+ * (cal2DUnsafe.h).  They run on the host mirrors and, in process source, on
+ * the GPU through the generic device program (calhip2D.h); the fused
+ * programs do not use them. */
 void calSetUnsafe2D(struct CALModel2D* ca2D);
 
 /* cal2D.c:333-440 — active-cell set. */

@@ -55,6 +55,42 @@ int calhipGetStats2D(struct CALModel2D* ca2D, struct calhipStats* out);
 int calhipEnableKernelTiming2D(struct CALModel2D* ca2D, int enable);
 int calhipGetKernelTime2D(struct CALModel2D* ca2D, double* total_ms, long long* launches);
 
+/* ------------------------------------------------- generic device programs */
+
+/* A model's own elementary processes, compiled for the GPU at run time.
+ * The source is C written against OpenCAL's cell API (calGet2D*, calGetX2D*,
+ * calSet2D*, calSetCurrent2D*, calInit2D*, the active-cell calls and the
+ * unsafe operations of cal2DUnsafe.h), the same code a CPU build of the
+ * model compiles; the variables its processes read (substate pointers,
+ * parameters) are declared CALHIP_GLOBAL, which is empty for host builds.
+ * A transition runs the processes in registration order, each followed by
+ * OpenCAL's update (cal2D.c:738-776).  Reads of a neighbour outside the
+ * grid's storage (flat boundary) give 0 and writes there are dropped. */
+#ifndef CALHIP_GLOBAL
+#define CALHIP_GLOBAL
+#endif
+
+/* Append C source to the model's process source (compiled on the first
+ * transition; one compilation per distinct source for the process). */
+int calhipAddProcessSource2D(struct CALModel2D* ca2D, const char* source);
+
+/* Run `process` (registered with calAddElementaryProcess2D) as the device
+ * function `device_name` of the source.  The model is stepped on the GPU
+ * once every registered process is bound. */
+int calhipBindProcess2D(struct CALModel2D* ca2D, CALCallbackFunc2D process, const char* device_name);
+
+/* Compile process source for the GPU without running it (no GPU is
+ * needed): 0 when it compiles with a kernel for each of the named device
+ * functions; otherwise -1 and the compiler's log in log (log_bytes long). */
+int calhipCheckProcessSource2D(const char* source, const char* const* names, int n_names, char* log,
+                               size_t log_bytes);
+
+/* Mirror the host variable at host_address (bytes long) into the
+ * CALHIP_GLOBAL `name` of the source before each transition.  Words that
+ * hold the address of one of this model's substates, or of the model, are
+ * given the address of its device view. */
+int calhipBindGlobal2D(struct CALModel2D* ca2D, const char* name, const void* host_address, size_t bytes);
+
 /* ------------------------------------------------------- device programs */
 
 /* Flow substates of SciddicaT are transient when the model's steering resets

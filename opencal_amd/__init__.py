@@ -116,6 +116,25 @@ def _declare_examples(E):
     E.sciara_model.restype = vp
     E.sciara_model.argtypes = [vp]
     E.sciara_destroy.argtypes = [vp]
+    # generic device program examples (examples/generic_models.c)
+    E.gsct_create.restype = vp
+    E.gsct_create.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_double, C.c_int, C.c_int]
+    E.gsct_run.argtypes = [vp]
+    E.gsct_read.argtypes = [vp, dp, dp, ip]
+    E.gsct_destroy.argtypes = [vp]
+    E.glife_create.restype = vp
+    E.glife_create.argtypes = [C.c_int, C.c_int, bp, C.c_int]
+    E.glife_run.argtypes = [vp]
+    E.glife_read.argtypes = [vp, bp]
+    E.glife_destroy.argtypes = [vp]
+    E.ggift_create.restype = vp
+    E.ggift_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    E.ggift_steps.argtypes = [vp, C.c_int]
+    E.ggift_apply_give.argtypes = [vp]
+    E.ggift_read.argtypes = [vp, ip, ip, dp]
+    E.ggift_table.argtypes = [vp, ip, C.c_int]
+    E.ggift_destroy.argtypes = [vp]
+    E.generic_check_sources.argtypes = [C.c_char_p, C.c_size_t]
 
 
 def last_error() -> tuple[int, str]:

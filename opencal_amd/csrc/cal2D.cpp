@@ -12,6 +12,10 @@
 #include "engine.hpp"
 
 extern "C" {
+#include "cal2DUnsafe.h"
+}
+
+extern "C" {
 #include "cal2DBuffer.h"
 }
 
@@ -469,10 +473,16 @@ int device_transition(ModelX* x, bool with_steering)
 // cal2D.c:717-734
 extern "C" void calApplyElementaryProcess2D(struct CALModel2D* m, CALCallbackFunc2D ep)
 {
-	(void)ep;
+	ModelX* x = X(m);
+	if (x->prog) {
+		if (ensure_device(x)) return;
+		const int r = x->prog->apply(x, ep);
+		if (r <= 0) return;
+	}
 	set_error(3, std::string("calApplyElementaryProcess2D: single-process sweeps are not provided by the ") +
-	                 (X(m)->prog ? X(m)->prog->name() : "(unbound)") +
-	                 " device program; use calGlobalTransitionFunction2D");
+	                 (x->prog ? x->prog->name() : "(unbound)") +
+	                 " device program; bind the processes to source (calhipBindProcess2D) or use "
+	                 "calGlobalTransitionFunction2D");
 }
 
 // ------------------------------------------------------------ accessors
@@ -529,6 +539,57 @@ static inline size_t nb_index(CALModel2D* m, int i, int j, int n)
 CAL_ACCESSORS(b, CALbyte)
 CAL_ACCESSORS(i, CALint)
 CAL_ACCESSORS(r, CALreal)
+
+// Unsafe operations (OpenCAL-OMP/source/cal2DUnsafe.c:17-275) on the host
+// mirrors: neighbour writes, reads of the next layer, additions to the next
+// layer.  The host phase is single-threaded, so the reference's cell locks
+// have nothing to guard.  (On the GPU these calls are part of the device API
+// of the generic program.)
+#define CAL_UNSAFE(SUF, T)                                                                              \
+	extern "C" void calInitX2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, int n, T v) \
+	{                                                                                                   \
+		if (ensure_host_for(X(m), Q)) return;                                                           \
+		const size_t c = nb_index(m, i, j, n);                                                          \
+		cur_of<T>(Q)[c] = v;                                                                            \
+		nxt_of<T>(Q)[c] = v;                                                                            \
+	}                                                                                                   \
+	extern "C" T calGetNext2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j)    \
+	{                                                                                                   \
+		if (ensure_host_for(X(m), Q)) return (T)0;                                                      \
+		return nxt_of<T>(Q)[idx(m, i, j)];                                                              \
+	}                                                                                                   \
+	extern "C" T calGetNextX2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, int n) \
+	{                                                                                                   \
+		if (ensure_host_for(X(m), Q)) return (T)0;                                                      \
+		return nxt_of<T>(Q)[nb_index(m, i, j, n)];                                                      \
+	}                                                                                                   \
+	extern "C" void calSetX2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, int n, T v) \
+	{                                                                                                   \
+		if (ensure_host_for(X(m), Q)) return;                                                           \
+		nxt_of<T>(Q)[nb_index(m, i, j, n)] = v;                                                         \
+	}                                                                                                   \
+	extern "C" void calSetCurrentX2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, int n, \
+	                                      T v)                                                          \
+	{                                                                                                   \
+		if (ensure_host_for(X(m), Q)) return;                                                           \
+		cur_of<T>(Q)[nb_index(m, i, j, n)] = v;                                                         \
+	}                                                                                                   \
+	extern "C" void calAddNext2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, T v) \
+	{                                                                                                   \
+		if (ensure_host_for(X(m), Q)) return;                                                           \
+		T* p = &nxt_of<T>(Q)[idx(m, i, j)];                                                             \
+		*p = (T)(*p + v);                                                                               \
+	}                                                                                                   \
+	extern "C" void calAddNextX2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, int n, T v) \
+	{                                                                                                   \
+		if (ensure_host_for(X(m), Q)) return;                                                           \
+		T* p = &nxt_of<T>(Q)[nb_index(m, i, j, n)];                                                     \
+		*p = (T)(*p + v);                                                                               \
+	}
+
+CAL_UNSAFE(b, CALbyte)
+CAL_UNSAFE(i, CALint)
+CAL_UNSAFE(r, CALreal)
 
 // ------------------------------------------------------------- finalize
 

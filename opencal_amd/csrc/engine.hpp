@@ -85,6 +85,10 @@ struct Program {
 	// Called before the device active set is read back: restores the
 	// row-major order of the list when the program keeps it unordered.
 	virtual int sync_active(ModelX*) { return 0; }
+	// calApplyElementaryProcess2D: one process over the grid (or the active
+	// list) without the update; 1 when the program offers no single-process
+	// sweep, <0 on error.
+	virtual int apply(ModelX*, CALCallbackFunc2D) { return 1; }
 };
 
 struct SubX {

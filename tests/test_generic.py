@@ -1,0 +1,172 @@
+"""The generic device program: a model's own elementary processes, written
+against OpenCAL's cell API, compiled for the GPU at run time and stepped
+with OpenCAL's per-process update (opencal_amd/examples/generic_models.c).
+
+CPU: the device API and every example source compile for gfx950; a bad
+source is reported with the compiler's message.  GPU: SciddicaT (full grid
+and active cells) and Life bit-exact against the oracle; the unsafe
+operations (calAddNextX2D*, calGetNext2D*) and calApplyElementaryProcess2D
+exact against a restatement in numpy on von Neumann, Moore and both
+hexagonal neighbourhoods, flat and toroidal."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import native
+from opencal_amd import examples, lib, synth
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_bp = C.POINTER(C.c_ubyte)
+
+
+def test_example_sources_compile():
+    E = examples()
+    log = C.create_string_buffer(16384)
+    assert E.generic_check_sources(log, 16384) == 0, log.value.decode()
+
+
+def test_bad_source_reports_the_compiler():
+    L = lib()
+    L.calhipCheckProcessSource2D.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_char_p, C.c_size_t]
+    log = C.create_string_buffer(8192)
+    names = (C.c_char_p * 1)(b"f")
+    src = b"void f(struct CALModel2D* ca, int i, int j) { calGet2Dr(ca, 0, i); }"
+    L.calhipClearError()
+    assert L.calhipCheckProcessSource2D(src, names, 1, log, 8192) != 0
+    assert b"calGet2Dr" in log.value
+    L.calhipClearError()
+    bad_name = (C.c_char_p * 1)(b"f; int x")
+    assert L.calhipCheckProcessSource2D(b"void f(struct CALModel2D* c, int i, int j) {}", bad_name, 1, log,
+                                        8192) != 0
+    L.calhipClearError()
+
+
+# ------------------------------------------------------------------- GPU
+
+def _gsct(z, h, p_r, p_eps, steps, active):
+    E = examples()
+    lib().calhipClearError()
+    hd = E.gsct_create(z.shape[0], z.shape[1], z.ctypes.data_as(_dp), h.ctypes.data_as(_dp), p_r, p_eps, steps,
+                       int(active))
+    assert hd, lib().calhipGetErrorString()
+    try:
+        assert E.gsct_run(hd) == 0, lib().calhipGetErrorString()
+        oz = np.empty_like(z)
+        oh = np.empty_like(h)
+        cnt = C.c_int(0)
+        assert E.gsct_read(hd, oz.ctypes.data_as(_dp), oh.ctypes.data_as(_dp), C.byref(cnt)) == 0
+        return oz, oh, cnt.value
+    finally:
+        E.gsct_destroy(hd)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("active", [False, True])
+@pytest.mark.parametrize("rows,cols,steps", [(48, 64, 120), (37, 29, 80)])
+def test_generic_sciddicat_bit_exact(rows, cols, steps, active):
+    print(f"case generic sciddicaT {rows}x{cols} steps={steps} active={active}", flush=True)
+    z, h, _ = synth.sciddicat_inputs(rows, cols, rows + cols, patch=8)
+    gz, gh, cnt = _gsct(z, h, 0.5, 0.001, steps, active)
+    oz, oh, counts = native.oracle_sciddicat(z, h, 0.5, 0.001, steps, active, counts=True)
+    assert np.array_equal(gz, oz)
+    assert np.array_equal(gh, oh), f"max |dh| = {np.abs(gh - oh).max()}"
+    if active:
+        assert cnt == counts[-1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows,cols,steps", [(64, 96, 150), (33, 47, 61)])
+def test_generic_life_bit_exact(rows, cols, steps):
+    print(f"case generic life {rows}x{cols} steps={steps}", flush=True)
+    E = examples()
+    a0 = synth.life_inputs(rows, cols, seed=rows + 3 * cols)
+    lib().calhipClearError()
+    hd = E.glife_create(rows, cols, a0.ctypes.data_as(_bp), steps)
+    assert hd, lib().calhipGetErrorString()
+    try:
+        assert E.glife_run(hd) == 0, lib().calhipGetErrorString()
+        got = np.empty_like(a0)
+        assert E.glife_read(hd, got.ctypes.data_as(_bp)) == 0
+    finally:
+        E.glife_destroy(hd)
+    assert np.array_equal(got, native.oracle_life(a0, steps))
+
+
+def _gift_expected(rows, cols, table, nb, torus, steps, apply_only=False):
+    """numpy restatement: q_next = q + sum over cells c of w(c) at each in-grid
+    neighbour of c; seen = q_next (after give) + q_after_give at neighbour 1;
+    acc += 0.25 * gifts.  Hexagonal: odd columns (hex) / odd rows (alt) use
+    the second table (n + 7)."""
+    q = np.zeros((rows, cols), np.int64)
+    seen = np.zeros((rows, cols), np.int64)
+    acc = np.zeros((rows, cols), np.float64)
+    n_x = len(table)
+    size_x = {1: 5, 2: 9, 3: 7, 4: 7}[nb]
+
+    def nbr(i, j, n):
+        if (nb == 3 and j % 2 == 1) or (nb == 4 and i % 2 == 1):
+            n += 7
+        ii, jj = i + table[n][0], j + table[n][1]
+        if torus:
+            ii = ii + rows if ii < 0 else (ii - rows if ii > rows - 1 else ii)
+            jj = jj + cols if jj < 0 else (jj - cols if jj > cols - 1 else jj)
+        lin = ii * cols + jj
+        return lin if 0 <= lin < rows * cols else None
+
+    assert n_x >= size_x
+    for _ in range(steps):
+        gift = np.zeros(rows * cols, np.int64)
+        for i in range(rows):
+            for j in range(cols):
+                w = 1 + (i * 7 + j * 3) % 5
+                for n in range(1, size_x):
+                    t = nbr(i, j, n)
+                    if t is not None:
+                        gift[t] += w
+        q = q + gift.reshape(rows, cols)
+        acc = acc + 0.25 * gift.reshape(rows, cols)
+        if apply_only:
+            continue
+        qf = q.reshape(-1)
+        new_seen = np.zeros(rows * cols, np.int64)
+        for i in range(rows):
+            for j in range(cols):
+                t = nbr(i, j, 1)
+                new_seen[i * cols + j] = qf[i * cols + j] + (qf[t] if t is not None else 0)
+        seen = new_seen.reshape(rows, cols)
+    return q, seen, acc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nb", [1, 2, 3, 4], ids=["von_neumann", "moore", "hex", "hex_alt"])
+@pytest.mark.parametrize("torus", [0, 1], ids=["flat", "toroidal"])
+def test_generic_unsafe_operations(nb, torus):
+    print(f"case generic unsafe nb={nb} torus={torus}", flush=True)
+    E = examples()
+    rows, cols, steps = 13, 18, 3
+    lib().calhipClearError()
+    hd = E.ggift_create(rows, cols, nb, torus)
+    assert hd, lib().calhipGetErrorString()
+    try:
+        tab = np.zeros(64, np.int32)
+        nx = E.ggift_table(hd, tab.ctypes.data_as(_ip), 32)
+        table = [(int(tab[2 * k]), int(tab[2 * k + 1])) for k in range(nx)]
+        assert E.ggift_steps(hd, steps) == 0, lib().calhipGetErrorString()
+        q = np.zeros((rows, cols), np.int32)
+        seen = np.zeros((rows, cols), np.int32)
+        acc = np.zeros((rows, cols), np.float64)
+        assert E.ggift_read(hd, q.ctypes.data_as(_ip), seen.ctypes.data_as(_ip), acc.ctypes.data_as(_dp)) == 0
+        eq, eseen, eacc = _gift_expected(rows, cols, table, nb, torus, steps)
+        assert np.array_equal(q, eq)
+        assert np.array_equal(seen, eseen)
+        assert np.array_equal(acc, eacc)   # sums of multiples of 0.25: exact in any order
+        # a single process through calApplyElementaryProcess2D + calUpdate2D
+        assert E.ggift_apply_give(hd) == 0, lib().calhipGetErrorString()
+        assert E.ggift_read(hd, q.ctypes.data_as(_ip), seen.ctypes.data_as(_ip), acc.ctypes.data_as(_dp)) == 0
+        eq2, _, eacc2 = _gift_expected(rows, cols, table, nb, torus, steps + 1)
+        eq_apply, _, _ = _gift_expected(rows, cols, table, nb, torus, 1, apply_only=True)
+        assert np.array_equal(q, eq + eq_apply)
+    finally:
+        E.ggift_destroy(hd)
