@@ -54,11 +54,104 @@ struct DevModelView {
 	int nsubs;
 };
 
-// Header names a process source may include; the API is already declared.
-const char* kHeaderNames[] = {"cal2D.h",  "calCommon.h", "cal2DUnsafe.h", "cal2DBuffer.h", "cal2DIO.h",
-                              "cal2DRun.h", "calhip2D.h", "math.h",       "stdlib.h",      "stdio.h",
-                              "string.h"};
+// Headers a process source may include.  The cell API is declared by the
+// device API above; these declare the host-side calls a model file also makes
+// (model definition, run loop, IO, libc) so that the whole file parses.  Host
+// code is never reached from a process kernel and is not compiled for the
+// GPU.  (hipRTC rejects empty header contents.)
+const char* kHdrCal2D = R"H(
+struct CALModel2D* calCADef2D(int, int, enum CALNeighborhood2D, enum CALSpaceBoundaryCondition, enum CALOptimization);
+void calSetUnsafe2D(struct CALModel2D*);
+void calUpdateActiveCells2D(struct CALModel2D*);
+struct CALCell2D* calAddNeighbor2D(struct CALModel2D*, int, int);
+struct CALSubstate2Db* calAddSubstate2Db(struct CALModel2D*);
+struct CALSubstate2Di* calAddSubstate2Di(struct CALModel2D*);
+struct CALSubstate2Dr* calAddSubstate2Dr(struct CALModel2D*);
+struct CALSubstate2Db* calAddSingleLayerSubstate2Db(struct CALModel2D*);
+struct CALSubstate2Di* calAddSingleLayerSubstate2Di(struct CALModel2D*);
+struct CALSubstate2Dr* calAddSingleLayerSubstate2Dr(struct CALModel2D*);
+CALCallbackFunc2D* calAddElementaryProcess2D(struct CALModel2D*, CALCallbackFunc2D);
+void calInitSubstate2Db(struct CALModel2D*, struct CALSubstate2Db*, CALbyte);
+void calInitSubstate2Di(struct CALModel2D*, struct CALSubstate2Di*, CALint);
+void calInitSubstate2Dr(struct CALModel2D*, struct CALSubstate2Dr*, CALreal);
+void calInitSubstateNext2Db(struct CALModel2D*, struct CALSubstate2Db*, CALbyte);
+void calInitSubstateNext2Di(struct CALModel2D*, struct CALSubstate2Di*, CALint);
+void calInitSubstateNext2Dr(struct CALModel2D*, struct CALSubstate2Dr*, CALreal);
+void calUpdateSubstate2Db(struct CALModel2D*, struct CALSubstate2Db*);
+void calUpdateSubstate2Di(struct CALModel2D*, struct CALSubstate2Di*);
+void calUpdateSubstate2Dr(struct CALModel2D*, struct CALSubstate2Dr*);
+void calApplyElementaryProcess2D(struct CALModel2D*, CALCallbackFunc2D);
+void calGlobalTransitionFunction2D(struct CALModel2D*);
+void calUpdate2D(struct CALModel2D*);
+void calFinalize2D(struct CALModel2D*);
+)H";
+const char* kHdrRun = R"H(
+struct CALRun2D {
+	struct CALModel2D* ca2D;
+	int step;
+	int initial_step;
+	int final_step;
+	enum CALUpdateMode UPDATE_MODE;
+	void (*init)(struct CALModel2D*);
+	void (*globalTransition)(struct CALModel2D*);
+	void (*steering)(struct CALModel2D*);
+	CALbyte (*stopCondition)(struct CALModel2D*);
+	void (*finalize)(struct CALModel2D*);
+};
+#define CAL_RUN_LOOP 0
+struct CALRun2D* calRunDef2D(struct CALModel2D*, int, int, enum CALUpdateMode);
+void calRunAddInitFunc2D(struct CALRun2D*, void (*)(struct CALModel2D*));
+void calRunAddGlobalTransitionFunc2D(struct CALRun2D*, void (*)(struct CALModel2D*));
+void calRunAddSteeringFunc2D(struct CALRun2D*, void (*)(struct CALModel2D*));
+void calRunAddStopConditionFunc2D(struct CALRun2D*, CALbyte (*)(struct CALModel2D*));
+void calRunAddFinalizeFunc2D(struct CALRun2D*, void (*)(struct CALModel2D*));
+void calRunInitSimulation2D(struct CALRun2D*);
+CALbyte calRunCAStep2D(struct CALRun2D*);
+void calRunFinalizeSimulation2D(struct CALRun2D*);
+void calRun2D(struct CALRun2D*);
+void calRunFinalize2D(struct CALRun2D*);
+)H";
+const char* kHdrIO = R"H(
+CALbyte calLoadSubstate2Db(struct CALModel2D*, struct CALSubstate2Db*, char*);
+CALbyte calLoadSubstate2Di(struct CALModel2D*, struct CALSubstate2Di*, char*);
+CALbyte calLoadSubstate2Dr(struct CALModel2D*, struct CALSubstate2Dr*, char*);
+CALbyte calSaveSubstate2Db(struct CALModel2D*, struct CALSubstate2Db*, char*);
+CALbyte calSaveSubstate2Di(struct CALModel2D*, struct CALSubstate2Di*, char*);
+CALbyte calSaveSubstate2Dr(struct CALModel2D*, struct CALSubstate2Dr*, char*);
+)H";
+const char* kHdrCalhip = R"H(
+int calhipAddProcessSource2D(struct CALModel2D*, const char*);
+int calhipBindProcess2D(struct CALModel2D*, CALCallbackFunc2D, const char*);
+int calhipBindGlobal2D(struct CALModel2D*, const char*, const void*, size_t);
+int calhipSyncHost2D(struct CALModel2D*);
+int calhipGetLastError(void);
+)H";
+const char* kHdrTime = R"H(
+typedef long time_t;
+time_t time(time_t*);
+)H";
+const char* kHdrStdio = R"H(
+typedef struct calhip_host_file FILE;
+FILE* fopen(const char*, const char*);
+int fclose(FILE*);
+int fprintf(FILE*, const char*, ...);
+int fscanf(FILE*, const char*, ...);
+)H";
+const char* kHdrStdlib = R"H(
+void exit(int);
+int atoi(const char*);
+double atof(const char*);
+int rand(void);
+void srand(unsigned);
+#define RAND_MAX 2147483647
+)H";
+const char* kHdrNone = "/* declared by the device API */\n";
+const char* kHeaderNames[] = {"cal2D.h",    "calCommon.h", "cal2DUnsafe.h", "cal2DBuffer.h", "cal2DIO.h", "cal2DRun.h",
+                              "calhip2D.h", "math.h",      "stdlib.h",      "stdio.h",       "string.h",  "time.h"};
+const char* kHeaderText[] = {kHdrCal2D,  kHdrNone, kHdrNone,   kHdrNone,  kHdrIO,   kHdrRun,
+                             kHdrCalhip, kHdrNone, kHdrStdlib, kHdrStdio, kHdrNone, kHdrTime};
 constexpr int kNumHeaders = sizeof(kHeaderNames) / sizeof(kHeaderNames[0]);
+static_assert(sizeof(kHeaderText) / sizeof(kHeaderText[0]) == kNumHeaders, "one text per header");
 
 std::mutex cache_mu;
 std::map<std::string, std::vector<char>>& code_cache()
@@ -77,11 +170,8 @@ int compile(const std::string& text, std::vector<char>& code)
 			return 0;
 		}
 	}
-	const char* hdr_src[kNumHeaders];
-	// (hipRTC rejects empty header contents)
-	for (int k = 0; k < kNumHeaders; ++k) hdr_src[k] = "/* declared by the device API */\n";
 	hiprtcProgram prog;
-	if (hiprtcCreateProgram(&prog, text.c_str(), "calhip_processes.hip", kNumHeaders, hdr_src, kHeaderNames) !=
+	if (hiprtcCreateProgram(&prog, text.c_str(), "calhip_processes.hip", kNumHeaders, kHeaderText, kHeaderNames) !=
 	    HIPRTC_SUCCESS) {
 		set_error(60, "calhip generic program: hiprtcCreateProgram failed");
 		return -1;
@@ -116,9 +206,10 @@ int compile(const std::string& text, std::vector<char>& code)
 std::string process_program_text(const std::string& source, const std::vector<std::string>& names)
 {
 	std::string t = kApi;
-	t += "\n#pragma clang force_cuda_host_device begin\n#line 1 \"process source\"\n";
+	// a model file's main() is host code: renamed so it is an ordinary function
+	t += "\n#define main calhip_model_main\n#pragma clang force_cuda_host_device begin\n#line 1 \"process source\"\n";
 	t += source;
-	t += "\n#pragma clang force_cuda_host_device end\n";
+	t += "\n#pragma clang force_cuda_host_device end\n#undef main\n";
 	for (size_t k = 0; k < names.size(); ++k) {
 		t += "extern \"C\" __global__ void __launch_bounds__(256) calhip_proc_" + std::to_string(k) +
 		     "(struct CALModel2D* m, const int* cells, const int* count)\n{\n"

@@ -33,6 +33,27 @@ def test_reference_program_runs_unchanged_on_the_engine(tmp_path):
     assert (data / "width_final.txt").read_text() == want
 
 
+BIN_G = os.path.join(native.REF_DIR, "dropin_sciddicaT_generic")
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_G), reason="dropin_sciddicaT_generic not built (needs the reference tree)")
+def test_reference_program_on_the_generic_device_program(tmp_path):
+    """The same program with its own elementary processes compiled for the
+    GPU at run time (its globals marked CALHIP_GLOBAL, five bind calls): the
+    shipped width_final.txt byte for byte."""
+    print("case dropin sciddicaT generic 610x496 4000 steps", flush=True)
+    data = tmp_path / "data"
+    data.mkdir()
+    for name, src in (("dem.txt", "sciddicaT_610x496_dem.txt.gz"), ("source.txt", "sciddicaT_610x496_source.txt.gz")):
+        with gzip.open(os.path.join(GOLDEN, src), "rb") as f, open(data / name, "wb") as g:
+            shutil.copyfileobj(f, g)
+    r = subprocess.run([BIN_G], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Simulation terminated" in r.stdout, r.stdout + r.stderr
+    want = gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
+    assert (data / "width_final.txt").read_text() == want
+
+
 BIN_SC = os.path.join(native.REF_DIR, "dropin_sciara")
 
 
