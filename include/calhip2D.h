@@ -60,12 +60,15 @@ int calhipGetKernelTime2D(struct CALModel2D* ca2D, double* total_ms, long long* 
 /* A model's own elementary processes, compiled for the GPU at run time.
  * The source is C written against OpenCAL's cell API (calGet2D*, calGetX2D*,
  * calSet2D*, calSetCurrent2D*, calInit2D*, the active-cell calls and the
- * unsafe operations of cal2DUnsafe.h), the same code a CPU build of the
- * model compiles; the variables its processes read (substate pointers,
- * parameters) are declared CALHIP_GLOBAL, which is empty for host builds.
- * A transition runs the processes in registration order, each followed by
- * OpenCAL's update (cal2D.c:738-776).  Reads of a neighbour outside the
- * grid's storage (flat boundary) give 0 and writes there are dropped. */
+ * unsafe operations of cal2DUnsafe.h), the same file a CPU build of the
+ * model compiles, unchanged: its functions are compiled for both sides and
+ * every file-scope variable also exists on the device.  The variables its
+ * processes read (substate pointers, parameters) are mirrored from the host
+ * with calhipBindGlobal2D.  A transition runs the processes in registration
+ * order, each followed by OpenCAL's update (cal2D.c:738-776).  Reads of a
+ * neighbour outside the grid's storage (flat boundary) give 0 and writes
+ * there are dropped.  CALHIP_GLOBAL marks a variable for the device
+ * explicitly; it is empty for host builds and optional. */
 #ifndef CALHIP_GLOBAL
 #define CALHIP_GLOBAL
 #endif
@@ -86,7 +89,7 @@ int calhipCheckProcessSource2D(const char* source, const char* const* names, int
                                size_t log_bytes);
 
 /* Mirror the host variable at host_address (bytes long) into the
- * CALHIP_GLOBAL `name` of the source before each transition.  Words that
+ * file-scope variable `name` of the source before each transition.  Words that
  * hold the address of one of this model's substates, or of the model, are
  * given the address of its device view. */
 int calhipBindGlobal2D(struct CALModel2D* ca2D, const char* name, const void* host_address, size_t bytes);

@@ -135,6 +135,41 @@ def _declare_examples(E):
     E.ggift_table.argtypes = [vp, ip, C.c_int]
     E.ggift_destroy.argtypes = [vp]
     E.generic_check_sources.argtypes = [C.c_char_p, C.c_size_t]
+    sbp = C.POINTER(C.c_byte)
+    E.gred_create.restype = vp
+    E.gred_create.argtypes = [C.c_int, C.c_int, sbp, ip, dp]
+    E.gred_reduce.restype = C.c_double
+    E.gred_reduce.argtypes = [vp, C.c_int, C.c_int]
+    E.gred_step.argtypes = [vp]
+    E.gred_on_device.argtypes = [vp]
+    E.gred_destroy.argtypes = [vp]
+    # 3D models (examples/generic_models3d.c)
+    E.gmod2_create.restype = vp
+    E.gmod2_create.argtypes = [C.c_int, C.c_int, C.c_int, bp, C.c_int]
+    E.gmod2_run.argtypes = [vp, bp]
+    E.gmod2_destroy.argtypes = [vp]
+    E.gheat_create.restype = vp
+    E.gheat_create.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_double, C.c_double, C.c_double, C.c_int, dp]
+    E.gheat_run.argtypes = [vp, dp, bp, ip]
+    E.gheat_reduce.restype = C.c_double
+    E.gheat_reduce.argtypes = [vp, C.c_int, C.c_int]
+    E.gheat_load.argtypes = [vp, dp, bp, ip]
+    E.gheat_on_device.argtypes = [vp]
+    E.gheat_step.argtypes = [vp]
+    E.gheat_destroy.argtypes = [vp]
+    E.ginf_create.restype = vp
+    E.ginf_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_int, C.c_int]
+    E.ginf_steps.argtypes = [vp, C.c_int]
+    E.ginf_read.argtypes = [vp, bp, bp, ip, ip, ip, C.c_int]
+    E.ginf_destroy.argtypes = [vp]
+    E.ggift3_create.restype = vp
+    E.ggift3_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    E.ggift3_steps.argtypes = [vp, C.c_int]
+    E.ggift3_apply_give.argtypes = [vp]
+    E.ggift3_read.argtypes = [vp, ip, ip, dp]
+    E.ggift3_table.argtypes = [vp, ip, C.c_int]
+    E.ggift3_destroy.argtypes = [vp]
+    E.generic3d_check_sources.argtypes = [C.c_char_p, C.c_size_t]
 
 
 def last_error() -> tuple[int, str]:

@@ -253,7 +253,8 @@ static void append(Pub*** arr, int* n, Pub* q)
 	(*n)++;
 }
 
-static SubX* new_sub(ModelX* x, char type, int elem, bool single)
+namespace calhip {
+SubX* new_substate(ModelX* x, char type, int elem, bool single)
 {
 	if (x->phase == Phase::Device && ensure_host(x)) return nullptr;
 	SubX* s = new (std::nothrow) SubX();
@@ -270,9 +271,13 @@ static SubX* new_sub(ModelX* x, char type, int elem, bool single)
 		delete s;
 		return nullptr;
 	}
+	s->owner = x;
 	x->subs.push_back(s);
 	return s;
 }
+}  // namespace calhip
+
+static SubX* new_sub(ModelX* x, char type, int elem, bool single) { return new_substate(x, type, elem, single); }
 
 // cal2D.c:470-552 (double-layer, registered for updates)
 extern "C" struct CALSubstate2Db* calAddSubstate2Db(struct CALModel2D* m)
@@ -452,6 +457,7 @@ extern "C" void calGlobalTransitionFunction2D(struct CALModel2D* m)
 		return;
 	}
 	if (ensure_device(x)) return;
+	device_changed(x);
 	if (x->prog->step(x) == 0) x->stats.steps++;
 }
 
@@ -464,6 +470,7 @@ int device_transition(ModelX* x, bool with_steering)
 		return -1;
 	}
 	if (ensure_device(x)) return -1;
+	device_changed(x);
 	int e = with_steering ? x->prog->step_and_steer(x) : x->prog->step(x);
 	if (e == 0) x->stats.steps++;
 	return e;
@@ -476,7 +483,8 @@ extern "C" void calApplyElementaryProcess2D(struct CALModel2D* m, CALCallbackFun
 	ModelX* x = X(m);
 	if (x->prog) {
 		if (ensure_device(x)) return;
-		const int r = x->prog->apply(x, ep);
+		device_changed(x);
+		const int r = x->prog->apply(x, (const void*)ep);
 		if (r <= 0) return;
 	}
 	set_error(3, std::string("calApplyElementaryProcess2D: single-process sweeps are not provided by the ") +
@@ -517,12 +525,12 @@ static inline size_t nb_index(CALModel2D* m, int i, int j, int n)
 	}                                                                                                \
 	extern "C" T calGet2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j)     \
 	{                                                                                                \
-		if (ensure_host_for(X(m), Q)) return (T)0;                                                   \
+		if (host_read(X(m), Q)) return (T)0;                                                         \
 		return cur_of<T>(Q)[idx(m, i, j)];                                                           \
 	}                                                                                                \
 	extern "C" T calGetX2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, int n) \
 	{                                                                                                \
-		if (ensure_host_for(X(m), Q)) return (T)0;                                                   \
+		if (host_read(X(m), Q)) return (T)0;                                                         \
 		return cur_of<T>(Q)[nb_index(m, i, j, n)];                                                   \
 	}                                                                                                \
 	extern "C" void calSet2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, int i, int j, T v) \
@@ -593,7 +601,8 @@ CAL_UNSAFE(r, CALreal)
 
 // ------------------------------------------------------------- finalize
 
-static void free_device(ModelX* x)
+namespace calhip {
+void release_device(ModelX* x)
 {
 	if (x->stream) hipStreamSynchronize(x->stream);
 	for (SubX* s : x->subs) {
@@ -605,6 +614,10 @@ static void free_device(ModelX* x)
 	if (x->d_cells) hipFree(x->d_cells);
 	if (x->d_count) hipFree(x->d_count);
 	if (x->d_block_counts) hipFree(x->d_block_counts);
+	if (x->d_red) hipFree(x->d_red);
+	if (x->d_stage) hipFree(x->d_stage);
+	x->d_red = x->d_stage = nullptr;
+	x->stage_bytes = 0;
 	x->d_flags = nullptr;
 	x->d_cells = x->d_count = x->d_block_counts = nullptr;
 	for (auto& p : x->timer.pending) {
@@ -615,13 +628,14 @@ static void free_device(ModelX* x)
 	if (x->stream) hipStreamDestroy(x->stream);
 	x->stream = nullptr;
 }
+}  // namespace calhip
 
 // cal2D.c:933-961 (also frees the substate records, which the reference leaks)
 extern "C" void calFinalize2D(struct CALModel2D* m)
 {
 	if (!m) return;
 	ModelX* x = X(m);
-	free_device(x);
+	release_device(x);
 	delete x->prog;
 	x->prog = nullptr;
 	free(m->A.flags);

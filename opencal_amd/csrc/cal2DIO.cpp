@@ -78,12 +78,12 @@ extern "C" void calfSaveMatrix2Dr(CALreal* M, int rows, int columns, FILE* f) { 
 	}                                                                                       \
 	extern "C" void calfSaveSubstate2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, FILE* f) \
 	{                                                                                       \
-		if (ensure_host_for(X(m), Q)) return;                                                      \
+		if (host_read(X(m), Q)) return;                                                     \
 		calfSaveMatrix2D##SUF(Q->current, m->rows, m->columns, f);                          \
 	}                                                                                       \
 	extern "C" CALbyte calSaveSubstate2D##SUF(struct CALModel2D* m, struct CALSubstate2D##SUF* Q, char* path) \
 	{                                                                                       \
-		if (ensure_host_for(X(m), Q)) return CAL_FALSE;                                            \
+		if (host_read(X(m), Q)) return CAL_FALSE;                                           \
 		return calSaveMatrix2D##SUF(Q->current, m->rows, m->columns, path);                 \
 	}
 

@@ -45,7 +45,10 @@ static void run_steering(struct CALRun2D* r)
 {
 	ModelX* x = X(r->ca2D);
 	if (x->prog && x->prog_steering == r->steering) {
-		if (ensure_device(x) == 0) x->prog->steer(x);
+		if (ensure_device(x) == 0) {
+			device_changed(x);
+			x->prog->steer(x);
+		}
 	} else {
 		r->steering(r->ca2D);
 	}
@@ -99,6 +102,7 @@ extern "C" void calRun2D(struct CALRun2D* r)
 		r->step = r->initial_step;
 		calGlobalTransitionFunction2D(r->ca2D);   // binds + uploads, one step
 		if (n > 1 && calhipGetLastError() == 0) {
+			device_changed(x);
 			if (x->prog->run_steps(x, n - 1) == 0) x->stats.steps += n - 1;
 		}
 		r->step = r->final_step + 1;

@@ -28,6 +28,7 @@
 extern "C" {
 #include "cal2D.h"
 #include "cal2DRun.h"
+#include "cal3D.h"
 #include "calhip2D.h"
 }
 
@@ -85,10 +86,10 @@ struct Program {
 	// Called before the device active set is read back: restores the
 	// row-major order of the list when the program keeps it unordered.
 	virtual int sync_active(ModelX*) { return 0; }
-	// calApplyElementaryProcess2D: one process over the grid (or the active
-	// list) without the update; 1 when the program offers no single-process
-	// sweep, <0 on error.
-	virtual int apply(ModelX*, CALCallbackFunc2D) { return 1; }
+	// calApplyElementaryProcess2D/3D: one process (a CALCallbackFunc2D or
+	// CALCallbackFunc3D) over the grid (or the active list) without the
+	// update; 1 when the program offers no single-process sweep, <0 on error.
+	virtual int apply(ModelX*, const void* /*process*/) { return 1; }
 };
 
 struct SubX {
@@ -110,6 +111,11 @@ struct SubX {
 	// does not use.  Its host array stays authoritative in every phase, so
 	// user code may even replace or free it as OpenCAL code does.
 	bool host_only = false;
+	// DEVICE phase: the host `current` mirror equals the device layer as of
+	// ModelX::dev_epoch == host_epoch (read-only host accessors then need no
+	// transfer; see host_read).
+	unsigned long long host_epoch = 0;
+	struct ModelX* owner = nullptr;
 
 	void* h_cur() const { return (void*)pub.r.current; }
 	void* h_nxt() const { return (void*)pub.r.next; }
@@ -145,11 +151,35 @@ struct ModelX {
 	calhipStats stats{};
 	Timer timer;
 
+	// reductions: block partials, and a staging copy of a host array
+	void* d_red = nullptr;
+	void* d_stage = nullptr;
+	size_t stage_bytes = 0;
+
+	// Incremented by every device operation that may change a substate while
+	// the model is in the DEVICE phase (see SubX::host_epoch).
+	unsigned long long dev_epoch = 1;
+
+	// A 3D model keeps its engine state in a ModelX whose private 2D shape is
+	// (slices*rows) x columns: the same linear index k*rows*columns +
+	// i*columns + j as OpenCAL's 3D buffers (cal3DBuffer.h:131-136).  pub3 is
+	// then the public 3D model; its active list (CALCell3D, (i,j,k) order) is
+	// converted at every upload/download.
+	CALModel3D* pub3 = nullptr;
+	std::vector<CALCallbackFunc3D> prog_eps3;   // 3D processes the program covers
+
 	size_t cells() const { return (size_t)pub.rows * (size_t)pub.columns; }
 	bool active() const { return pub.OPTIMIZATION == CAL_OPT_ACTIVE_CELLS; }
 };
 
 inline ModelX* X(CALModel2D* m) { return reinterpret_cast<ModelX*>(m); }
+
+// A 3D model: the public struct, then the engine state (see ModelX::pub3).
+struct Model3X {
+	CALModel3D pub;
+	ModelX core;
+};
+inline Model3X* X3(CALModel3D* m) { return reinterpret_cast<Model3X*>(m); }
 inline SubX* S(void* q) { return reinterpret_cast<SubX*>(q); }
 
 // ---- error handling (never exits the process) ----
@@ -158,7 +188,12 @@ int check_hip(hipError_t e, const char* what);
 #define CALHIP_CHECK(call) ::calhip::check_hip((call), #call)
 
 // ---- residency ----
+int ensure_stream(ModelX* m);  // the model's stream (created on first use)
 int ensure_host(ModelX* m);    // DEVICE -> HOST (no-op in HOST)
+// Read-only host access to `current` of q: in the DEVICE phase only that
+// layer is copied back (once per device epoch) and the model stays there.
+int host_read(ModelX* m, void* q);
+inline void device_changed(ModelX* m) { m->dev_epoch++; }
 // ensure_host, unless the call touches only the host-only substate q
 inline int ensure_host_for(ModelX* m, void* q) { return S(q)->host_only ? 0 : ensure_host(m); }
 int ensure_device(ModelX* m);  // HOST -> DEVICE (no-op in DEVICE)
@@ -173,10 +208,16 @@ void flip(SubX* s);
 // program's steering twin (cal2D.cpp); 0 on success.
 int device_transition(ModelX* m, bool with_steering);
 
+// ---- shared by the 2D and 3D APIs (cal2D.cpp) ----
+SubX* new_substate(ModelX* x, char type, int elem, bool single);
+void release_device(ModelX* x);   // device buffers, stream, timers
+
 // ---- kernel launchers (defined in the .hip files) ----
 namespace k {
 int fill(ModelX* m, void* buf, int elem, double v, bool active_only);
 int copy(ModelX* m, void* dst, const void* src, size_t bytes);
+// next -> current at the active list's cells (the whole grid when it is empty)
+int copy_active(ModelX* m, void* dst, const void* src, int elem);
 int rebuild_active(ModelX* m);
 int active_count_host(ModelX* m, int* out);  // synchronous read of count[0]
 }  // namespace k

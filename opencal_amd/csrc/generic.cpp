@@ -8,10 +8,12 @@
 // active cells, the unsafe operations) and is compiled with hipRTC together
 // with the device side of that API (generic_api.inc) and one kernel per
 // process that calls it for every cell of the grid, or every cell of the
-// active list.  Globals the processes read are declared CALHIP_GLOBAL and
-// mirrored from the host (calhipBindGlobal2D); pointer-sized words that hold
-// the host address of one of the model's substates, or of the model, become
-// the address of its device view.
+// active list.  Every file-scope variable of the source is also a device
+// variable; those the processes read are mirrored from the host
+// (calhipBindGlobal2D/3D) before each transition, and pointer-sized words
+// that hold the host address of one of the model's substates, or of the
+// model, become the address of its device view.  3D models (CALModel3D,
+// ModelX::pub3) use the device API of generic_api3.inc and 3D cell indices.
 //
 // A transition runs the processes in registration order, each followed by
 // OpenCAL's update (OpenCAL-OMP/source/cal2D.c:738-776): the active set is
@@ -37,8 +39,14 @@ namespace calhip {
 
 namespace {
 
-const char* kApi =
+const char* kApiCommon =
+#include "generic_api_common.inc"
+    ;
+const char* kApi2 =
 #include "generic_api.inc"
+    ;
+const char* kApi3 =
+#include "generic_api3.inc"
     ;
 
 // Host image of the device model view (the struct CALModel2D of
@@ -49,6 +57,20 @@ struct DevModelView {
 	CALCell2D* X;
 	int sizeof_X;
 	int X_id;
+	int nx;
+	void** subs;
+	int nsubs;
+};
+
+// Host image of the 3D device model view (struct CALModel3D of
+// generic_api3.inc).
+struct DevModelView3 {
+	int rows, columns, slices, T, OPTIMIZATION;
+	CALActiveCells3D A;
+	CALCell3D* X;
+	int sizeof_X;
+	int X_id;
+	int is_safe;
 	int nx;
 	void** subs;
 	int nsubs;
@@ -146,12 +168,112 @@ void srand(unsigned);
 #define RAND_MAX 2147483647
 )H";
 const char* kHdrNone = "/* declared by the device API */\n";
-const char* kHeaderNames[] = {"cal2D.h",    "calCommon.h", "cal2DUnsafe.h", "cal2DBuffer.h", "cal2DIO.h", "cal2DRun.h",
-                              "calhip2D.h", "math.h",      "stdlib.h",      "stdio.h",       "string.h",  "time.h"};
-const char* kHeaderText[] = {kHdrCal2D,  kHdrNone, kHdrNone,   kHdrNone,  kHdrIO,   kHdrRun,
-                             kHdrCalhip, kHdrNone, kHdrStdlib, kHdrStdio, kHdrNone, kHdrTime};
-constexpr int kNumHeaders = sizeof(kHeaderNames) / sizeof(kHeaderNames[0]);
-static_assert(sizeof(kHeaderText) / sizeof(kHeaderText[0]) == kNumHeaders, "one text per header");
+const char* kHdrCal3D = R"H(
+struct CALModel3D* calCADef3D(int, int, int, enum CALNeighborhood3D, enum CALSpaceBoundaryCondition, enum CALOptimization);
+void calSetUnsafe3D(struct CALModel3D*);
+void calUpdateActiveCells3D(struct CALModel3D*);
+struct CALCell3D* calAddNeighbor3D(struct CALModel3D*, int, int, int);
+struct CALSubstate3Db* calAddSubstate3Db(struct CALModel3D*);
+struct CALSubstate3Di* calAddSubstate3Di(struct CALModel3D*);
+struct CALSubstate3Dr* calAddSubstate3Dr(struct CALModel3D*);
+struct CALSubstate3Db* calAddSingleLayerSubstate3Db(struct CALModel3D*);
+struct CALSubstate3Di* calAddSingleLayerSubstate3Di(struct CALModel3D*);
+struct CALSubstate3Dr* calAddSingleLayerSubstate3Dr(struct CALModel3D*);
+CALCallbackFunc3D* calAddElementaryProcess3D(struct CALModel3D*, CALCallbackFunc3D);
+void calInitSubstate3Db(struct CALModel3D*, struct CALSubstate3Db*, CALbyte);
+void calInitSubstate3Di(struct CALModel3D*, struct CALSubstate3Di*, CALint);
+void calInitSubstate3Dr(struct CALModel3D*, struct CALSubstate3Dr*, CALreal);
+void calInitSubstateNext3Db(struct CALModel3D*, struct CALSubstate3Db*, CALbyte);
+void calInitSubstateNext3Di(struct CALModel3D*, struct CALSubstate3Di*, CALint);
+void calInitSubstateNext3Dr(struct CALModel3D*, struct CALSubstate3Dr*, CALreal);
+void calUpdateSubstate3Db(struct CALModel3D*, struct CALSubstate3Db*);
+void calUpdateSubstate3Di(struct CALModel3D*, struct CALSubstate3Di*);
+void calUpdateSubstate3Dr(struct CALModel3D*, struct CALSubstate3Dr*);
+void calApplyElementaryProcess3D(struct CALModel3D*, CALCallbackFunc3D);
+void calGlobalTransitionFunction3D(struct CALModel3D*);
+void calUpdate3D(struct CALModel3D*);
+void calFinalize3D(struct CALModel3D*);
+)H";
+const char* kHdrRun3 = R"H(
+struct CALRun3D {
+	struct CALModel3D* ca3D;
+	int step;
+	int initial_step;
+	int final_step;
+	enum CALUpdateMode UPDATE_MODE;
+	void (*init)(struct CALModel3D*);
+	void (*globalTransition)(struct CALModel3D*);
+	void (*steering)(struct CALModel3D*);
+	CALbyte (*stopCondition)(struct CALModel3D*);
+	void (*finalize)(struct CALModel3D*);
+};
+#define CAL_RUN_LOOP 0
+struct CALRun3D* calRunDef3D(struct CALModel3D*, int, int, enum CALUpdateMode);
+void calRunAddInitFunc3D(struct CALRun3D*, void (*)(struct CALModel3D*));
+void calRunAddGlobalTransitionFunc3D(struct CALRun3D*, void (*)(struct CALModel3D*));
+void calRunAddSteeringFunc3D(struct CALRun3D*, void (*)(struct CALModel3D*));
+void calRunAddStopConditionFunc3D(struct CALRun3D*, CALbyte (*)(struct CALModel3D*));
+void calRunAddFinalizeFunc3D(struct CALRun3D*, void (*)(struct CALModel3D*));
+void calRunInitSimulation3D(struct CALRun3D*);
+CALbyte calRunCAStep3D(struct CALRun3D*);
+void calRunFinalizeSimulation3D(struct CALRun3D*);
+void calRun3D(struct CALRun3D*);
+void calRunFinalize3D(struct CALRun3D*);
+)H";
+const char* kHdrIO3 = R"H(
+CALbyte calLoadSubstate3Db(struct CALModel3D*, struct CALSubstate3Db*, char*);
+CALbyte calLoadSubstate3Di(struct CALModel3D*, struct CALSubstate3Di*, char*);
+CALbyte calLoadSubstate3Dr(struct CALModel3D*, struct CALSubstate3Dr*, char*);
+CALbyte calSaveSubstate3Db(struct CALModel3D*, struct CALSubstate3Db*, char*);
+CALbyte calSaveSubstate3Di(struct CALModel3D*, struct CALSubstate3Di*, char*);
+CALbyte calSaveSubstate3Dr(struct CALModel3D*, struct CALSubstate3Dr*, char*);
+)H";
+const char* kHdrCalhip3 = R"H(
+int calhipAddProcessSource3D(struct CALModel3D*, const char*);
+int calhipBindProcess3D(struct CALModel3D*, CALCallbackFunc3D, const char*);
+int calhipBindGlobal3D(struct CALModel3D*, const char*, const void*, size_t);
+int calhipSyncHost3D(struct CALModel3D*);
+int calhipGetLastError(void);
+)H";
+
+// Host declarations of the reductions (cal2DReduction.h / cal3DReduction.h).
+std::string reduction_header(int dims)
+{
+	static const char* ops[] = {"Max",       "Min",        "Sum",       "Prod",       "LogicalAnd",
+	                            "BinaryAnd", "LogicalOr",  "BinaryOr",  "LogicalXor", "BinaryXor"};
+	static const char* suf[] = {"b", "i", "r"};
+	static const char* ty[] = {"CALbyte", "CALint", "CALreal"};
+	const std::string d = std::to_string(dims);
+	std::string h = "\nenum REDUCTION_OPERATION { REDUCTION_NONE = 0, REDUCTION_MAX, REDUCTION_MIN, REDUCTION_SUM, "
+	                "REDUCTION_PROD, REDUCTION_LOGICAL_AND, REDUCTION_BINARY_AND, REDUCTION_LOGICAL_OR, "
+	                "REDUCTION_BINARY_OR, REDUCTION_LOGICAL_XOR, REDUCTION_BINARY_XOR };\n";
+	for (int t = 0; t < 3; ++t) {
+		const std::string sig = std::string("(struct CALModel") + d + "D*, struct CALSubstate" + d + "D" + suf[t] + "*";
+		h += std::string(ty[t]) + " calReductionOperation" + d + "D" + suf[t] + sig + ", enum REDUCTION_OPERATION);\n";
+		for (const char* op : ops) h += std::string(ty[t]) + " calReductionCompute" + op + d + "D" + suf[t] + sig + ");\n";
+	}
+	return h;
+}
+
+struct HeaderSet {
+	std::vector<const char*> names, texts;
+};
+
+const HeaderSet& headers(int dims)
+{
+	static const std::string red2 = reduction_header(2), red3 = reduction_header(3);
+	static const HeaderSet h2 = {
+	    {"cal2D.h", "calCommon.h", "cal2DUnsafe.h", "cal2DBuffer.h", "cal2DIO.h", "cal2DRun.h", "cal2DReduction.h",
+	     "calhip2D.h", "math.h", "stdlib.h", "stdio.h", "string.h", "time.h"},
+	    {kHdrCal2D, kHdrNone, kHdrNone, kHdrNone, kHdrIO, kHdrRun, red2.c_str(), kHdrCalhip, kHdrNone, kHdrStdlib,
+	     kHdrStdio, kHdrNone, kHdrTime}};
+	static const HeaderSet h3 = {
+	    {"cal3D.h", "calCommon.h", "cal3DUnsafe.h", "cal3DBuffer.h", "cal3DIO.h", "cal3DRun.h", "cal3DReduction.h",
+	     "calhip3D.h", "math.h", "stdlib.h", "stdio.h", "string.h", "time.h"},
+	    {kHdrCal3D, kHdrNone, kHdrNone, kHdrNone, kHdrIO3, kHdrRun3, red3.c_str(), kHdrCalhip3, kHdrNone, kHdrStdlib,
+	     kHdrStdio, kHdrNone, kHdrTime}};
+	return dims == 3 ? h3 : h2;
+}
 
 std::mutex cache_mu;
 std::map<std::string, std::vector<char>>& code_cache()
@@ -160,7 +282,7 @@ std::map<std::string, std::vector<char>>& code_cache()
 	return *c;
 }
 
-int compile(const std::string& text, std::vector<char>& code)
+int compile(const std::string& text, int dims, std::vector<char>& code)
 {
 	{
 		std::lock_guard<std::mutex> lk(cache_mu);
@@ -171,15 +293,19 @@ int compile(const std::string& text, std::vector<char>& code)
 		}
 	}
 	hiprtcProgram prog;
-	if (hiprtcCreateProgram(&prog, text.c_str(), "calhip_processes.hip", kNumHeaders, kHeaderText, kHeaderNames) !=
-	    HIPRTC_SUCCESS) {
+	const HeaderSet& hs = headers(dims);
+	if (hiprtcCreateProgram(&prog, text.c_str(), "calhip_processes.hip", (int)hs.names.size(), hs.texts.data(),
+	                        hs.names.data()) != HIPRTC_SUCCESS) {
 		set_error(60, "calhip generic program: hiprtcCreateProgram failed");
 		return -1;
 	}
 	// -ffp-contract=off: no fused multiply-add, the processes' arithmetic is
-	// the reference's operation by operation
+	// the reference's operation by operation.  -fgpu-defer-diag: host code of
+	// the model file (which uses host-only globals such as a FILE*) is
+	// diagnosed only if a kernel reaches it.
 	const char* opts[] = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-std=c++17", "-Xclang",
-	                      "-target-feature",        "-Xclang", "-packed-fp32-ops", "-Wno-everything"};
+	                      "-target-feature",        "-Xclang", "-packed-fp32-ops", "-Wno-everything",
+	                      "-fgpu-defer-diag"};
 	const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
 	if (r != HIPRTC_SUCCESS) {
 		size_t ls = 0;
@@ -203,14 +329,33 @@ int compile(const std::string& text, std::vector<char>& code)
 // The text hipRTC compiles: the device API, the model's source (its
 // functions become host-and-device functions), and one kernel per process
 // calling it for every cell of the grid or of the active list.
-std::string process_program_text(const std::string& source, const std::vector<std::string>& names)
+std::string process_program_text(const std::string& source, const std::vector<std::string>& names, int dims)
 {
-	std::string t = kApi;
-	// a model file's main() is host code: renamed so it is an ordinary function
-	t += "\n#define main calhip_model_main\n#pragma clang force_cuda_host_device begin\n#line 1 \"process source\"\n";
+	std::string t = kApiCommon;
+	t += dims == 3 ? kApi3 : kApi2;
+	// A model file's main() is host code: renamed so it is an ordinary
+	// function.  Every function of the file becomes host-and-device, every
+	// file-scope variable a device variable (its initial value, or the host
+	// value mirrored by calhipBindGlobal*), so the file compiles unchanged.
+	t += "\n#define main calhip_model_main\n#pragma clang force_cuda_host_device begin\n"
+	     "#pragma clang attribute push (__attribute__((device)), apply_to = variable(is_global))\n"
+	     "#line 1 \"process source\"\n";
 	t += source;
-	t += "\n#pragma clang force_cuda_host_device end\n#undef main\n";
-	for (size_t k = 0; k < names.size(); ++k) {
+	t += "\n#pragma clang attribute pop\n#pragma clang force_cuda_host_device end\n#undef main\n";
+	for (size_t k = 0; k < names.size() && dims == 3; ++k) {
+		t += "extern \"C\" __global__ void __launch_bounds__(256) calhip_proc_" + std::to_string(k) +
+		     "(struct CALModel3D* m, const int* cells, const int* count)\n{\n"
+		     "\tconst int cnt = count ? count[0] : 0;\n"
+		     "\tconst unsigned long long cols = (unsigned long long)m->columns;\n"
+		     "\tconst unsigned long long plane = (unsigned long long)m->rows * cols;\n"
+		     "\tconst unsigned long long total = cnt > 0 ? (unsigned long long)cnt : plane * m->slices;\n"
+		     "\tfor (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;"
+		     " t += (unsigned long long)gridDim.x * blockDim.x) {\n"
+		     "\t\tconst unsigned long long c = cnt > 0 ? (unsigned long long)cells[t] : t;\n"
+		     "\t\tconst unsigned long long r = c % plane;\n"
+		     "\t\t" + names[k] + "(m, (int)(r / cols), (int)(r % cols), (int)(c / plane));\n\t}\n}\n";
+	}
+	for (size_t k = 0; k < names.size() && dims != 3; ++k) {
 		t += "extern \"C\" __global__ void __launch_bounds__(256) calhip_proc_" + std::to_string(k) +
 		     "(struct CALModel2D* m, const int* cells, const int* count)\n{\n"
 		     "\tconst int cnt = count ? count[0] : 0;\n"
@@ -245,9 +390,10 @@ bool valid_identifier(const char* s)
 }  // namespace
 
 struct GenericProgram : Program {
+	int dims = 2;   // 3 for a CALModel3D (ModelX::pub3)
 	std::string source;
 	struct Bound {
-		CALCallbackFunc2D host;
+		const void* host;   // CALCallbackFunc2D or CALCallbackFunc3D
 		std::string name;
 	};
 	std::vector<Bound> procs;
@@ -264,7 +410,7 @@ struct GenericProgram : Program {
 	std::vector<hipFunction_t> fns;   // per entry of procs
 	std::string built_text;
 	void* d_view = nullptr;
-	CALCell2D* d_X = nullptr;
+	void* d_X = nullptr;
 	void** d_subs = nullptr;             // {current, next} per substate, in ModelX::subs order
 	std::vector<void*> subs_uploaded;
 
@@ -282,7 +428,7 @@ struct GenericProgram : Program {
 	{
 		std::vector<std::string> names;
 		for (const Bound& b : procs) names.push_back(b.name);
-		return process_program_text(source, names);
+		return process_program_text(source, names, dims);
 	}
 
 	size_t nsubs_built = 0;   // substates the device view was built for
@@ -292,7 +438,7 @@ struct GenericProgram : Program {
 		const std::string text = program_text();
 		if (module && text == built_text && m->subs.size() == nsubs_built) return 0;
 		std::vector<char> code;
-		if (compile(text, code)) return -1;
+		if (compile(text, dims, code)) return -1;
 		if (module) (void)hipModuleUnload(module);
 		module = nullptr;
 		fns.clear();
@@ -308,7 +454,7 @@ struct GenericProgram : Program {
 			size_t bytes = 0;
 			if (hipModuleGetGlobal(&p, &bytes, module, g.name.c_str()) != hipSuccess || bytes != g.bytes) {
 				(void)hipGetLastError();
-				set_error(62, "calhip generic program: the process source declares no CALHIP_GLOBAL '" + g.name +
+				set_error(62, "calhip generic program: the process source declares no global '" + g.name +
 				                  "' of " + std::to_string(g.bytes) + " bytes");
 				return -1;
 			}
@@ -316,13 +462,19 @@ struct GenericProgram : Program {
 			g.last.clear();
 		}
 		built_text = text;
+		if (d_subs) (void)hipFree(d_subs);
+		d_subs = nullptr;
+		if (CALHIP_CHECK(hipMalloc(&d_subs, sizeof(void*) * 2 * (m->subs.size() + 1)))) return -1;
+		subs_uploaded.clear();
+		nsubs_built = m->subs.size();
+		if (d_X) (void)hipFree(d_X);
+		d_X = nullptr;
+		if (dims == 3) return build_view3(m);
 		// model view and neighbourhood
 		CALModel2D* p = &m->pub;
 		// hexagonal tables hold the even and the odd table (cal2D.c:75-166)
 		const bool hex = p->X_id == CAL_HEXAGONAL_NEIGHBORHOOD_2D || p->X_id == CAL_HEXAGONAL_NEIGHBORHOOD_ALT_2D;
 		const int nx = p->sizeof_X + (hex ? CAL_HEXAGONAL_SHIFT : 0);
-		if (d_X) (void)hipFree(d_X);
-		d_X = nullptr;
 		if (CALHIP_CHECK(hipMalloc(&d_X, sizeof(CALCell2D) * (size_t)(nx > 0 ? nx : 1)))) return -1;
 		if (nx > 0 && CALHIP_CHECK(hipMemcpy(d_X, p->X, sizeof(CALCell2D) * (size_t)nx, hipMemcpyHostToDevice)))
 			return -1;
@@ -334,18 +486,40 @@ struct GenericProgram : Program {
 		v.OPTIMIZATION = (int)p->OPTIMIZATION;
 		v.A.flags = (CALbyte*)m->d_flags;
 		v.A.num_threads = 1;
-		v.X = d_X;
+		v.X = (CALCell2D*)d_X;
 		v.sizeof_X = p->sizeof_X;
 		v.X_id = (int)p->X_id;
 		v.nx = nx;
-		if (d_subs) (void)hipFree(d_subs);
-		d_subs = nullptr;
-		if (CALHIP_CHECK(hipMalloc(&d_subs, sizeof(void*) * 2 * (m->subs.size() + 1)))) return -1;
-		subs_uploaded.clear();
 		v.subs = d_subs;
 		v.nsubs = (int)m->subs.size();
-		nsubs_built = m->subs.size();
 		if (!d_view && CALHIP_CHECK(hipMalloc(&d_view, sizeof(DevModelView)))) return -1;
+		if (CALHIP_CHECK(hipMemcpy(d_view, &v, sizeof v, hipMemcpyHostToDevice))) return -1;
+		return 0;
+	}
+
+	int build_view3(ModelX* m)
+	{
+		const CALModel3D* p = m->pub3;
+		const int nx = p->sizeof_X;
+		if (CALHIP_CHECK(hipMalloc(&d_X, sizeof(CALCell3D) * (size_t)(nx > 0 ? nx : 1)))) return -1;
+		if (nx > 0 && CALHIP_CHECK(hipMemcpy(d_X, p->X, sizeof(CALCell3D) * (size_t)nx, hipMemcpyHostToDevice)))
+			return -1;
+		DevModelView3 v = {};
+		v.rows = p->rows;
+		v.columns = p->columns;
+		v.slices = p->slices;
+		v.T = (int)p->T;
+		v.OPTIMIZATION = (int)p->OPTIMIZATION;
+		v.A.flags = (CALbyte*)m->d_flags;
+		v.A.num_threads = 1;
+		v.X = (CALCell3D*)d_X;
+		v.sizeof_X = p->sizeof_X;
+		v.X_id = (int)p->X_id;
+		v.is_safe = (int)p->is_safe;
+		v.nx = nx;
+		v.subs = d_subs;
+		v.nsubs = (int)m->subs.size();
+		if (!d_view && CALHIP_CHECK(hipMalloc(&d_view, sizeof(DevModelView3)))) return -1;
 		if (CALHIP_CHECK(hipMemcpy(d_view, &v, sizeof v, hipMemcpyHostToDevice))) return -1;
 		return 0;
 	}
@@ -374,7 +548,7 @@ struct GenericProgram : Program {
 				memcpy(&w, b.data() + off, sizeof w);
 				if (!w) continue;
 				uintptr_t to = 0;
-				if (w == (uintptr_t)&m->pub) {
+				if (w == (uintptr_t)&m->pub || (m->pub3 && w == (uintptr_t)m->pub3)) {
 					to = (uintptr_t)d_view;
 				} else {
 					for (size_t k = 0; k < m->subs.size(); ++k)
@@ -403,7 +577,7 @@ struct GenericProgram : Program {
 		return 0;
 	}
 
-	int index_of(CALCallbackFunc2D ep) const
+	int index_of(const void* ep) const
 	{
 		for (size_t k = 0; k < procs.size(); ++k)
 			if (procs[k].host == ep) return (int)k;
@@ -428,13 +602,19 @@ struct GenericProgram : Program {
 		return 0;
 	}
 
-	// OpenCAL's update after a process: rebuild, then next -> current.
+	// OpenCAL's update after a process (cal2D.c:758-776, cal3D.c:697-715):
+	// rebuild the active set, then next -> current for every double-layer
+	// substate, at the active cells in active mode (the whole grid while
+	// the list is empty).  In active mode the layers may then differ outside
+	// the set, as in the reference.
 	int update(ModelX* m)
 	{
 		if (device_rebuild_active(m)) return -1;
 		for (SubX* s : m->subs) {
 			if (s->single) continue;
-			if (k::copy(m, s->d_cur, s->d_nxt, m->cells() * (size_t)s->elem)) return -1;
+			const int e = m->active() ? k::copy_active(m, s->d_cur, s->d_nxt, s->elem)
+			                          : k::copy(m, s->d_cur, s->d_nxt, m->cells() * (size_t)s->elem);
+			if (e) return -1;
 			s->nxt_stale = false;
 			s->nxt_dirty = false;
 		}
@@ -444,9 +624,10 @@ struct GenericProgram : Program {
 	int step(ModelX* m) override
 	{
 		if (prepare(m)) return -1;
-		CALModel2D* p = &m->pub;
-		for (int e = 0; e < p->num_of_elementary_processes; ++e) {
-			const int k = index_of(p->elementary_processes[e]);
+		const int n_ep = m->pub3 ? m->pub3->num_of_elementary_processes : m->pub.num_of_elementary_processes;
+		for (int e = 0; e < n_ep; ++e) {
+			const int k = index_of(m->pub3 ? (const void*)m->pub3->elementary_processes[e]
+			                               : (const void*)m->pub.elementary_processes[e]);
 			if (k < 0) {
 				set_error(63, "calhip generic program: a process is not bound to device source");
 				return -1;
@@ -458,7 +639,7 @@ struct GenericProgram : Program {
 
 	// calApplyElementaryProcess2D: the sweep alone; the caller's calUpdate2D
 	// flips the layers (each next layer was equal to current before it).
-	int apply(ModelX* m, CALCallbackFunc2D ep) override
+	int apply(ModelX* m, const void* ep) override
 	{
 		const int k = index_of(ep);
 		if (k < 0) {
@@ -481,8 +662,10 @@ GenericProgram* generic_of(ModelX* m, bool create)
 	if (ensure_host(m)) return nullptr;   // leaving another program: its state is written back first
 	delete m->prog;
 	g = new GenericProgram();
+	g->dims = m->pub3 ? 3 : 2;
 	m->prog = g;
 	m->prog_eps.clear();
+	m->prog_eps3.clear();
 	m->prog_steering = nullptr;
 	for (SubX* s : m->subs) s->host_only = false;
 	return g;
@@ -494,26 +677,27 @@ GenericProgram* generic_of(ModelX* m, bool create)
 
 using namespace calhip;
 
-extern "C" int calhipAddProcessSource2D(struct CALModel2D* ca2D, const char* source)
+namespace {
+
+int add_source(ModelX* m, const char* source, const char* fn)
 {
-	if (!ca2D || !source) {
-		set_error(64, "calhipAddProcessSource2D: missing argument");
+	if (!m || !source) {
+		set_error(64, std::string(fn) + ": missing argument");
 		return -1;
 	}
-	GenericProgram* g = generic_of(X(ca2D), true);
+	GenericProgram* g = generic_of(m, true);
 	if (!g) return -1;
 	g->source += source;
 	g->source += "\n";
 	return 0;
 }
 
-extern "C" int calhipBindProcess2D(struct CALModel2D* ca2D, CALCallbackFunc2D process, const char* device_name)
+int bind_process(ModelX* m, const void* process, const char* device_name, const char* fn)
 {
-	if (!ca2D || !process || !valid_identifier(device_name)) {
-		set_error(64, "calhipBindProcess2D: give the process and the C name of its device function");
+	if (!m || !process || !valid_identifier(device_name)) {
+		set_error(64, std::string(fn) + ": give the process and the C name of its device function");
 		return -1;
 	}
-	ModelX* m = X(ca2D);
 	GenericProgram* g = generic_of(m, true);
 	if (!g) return -1;
 	const int k = g->index_of(process);
@@ -523,18 +707,28 @@ extern "C" int calhipBindProcess2D(struct CALModel2D* ca2D, CALCallbackFunc2D pr
 		g->procs.push_back({process, device_name});
 	// the program covers the model's processes in registration order
 	m->prog_eps.clear();
-	for (int e = 0; e < ca2D->num_of_elementary_processes; ++e)
-		if (g->index_of(ca2D->elementary_processes[e]) >= 0) m->prog_eps.push_back(ca2D->elementary_processes[e]);
+	m->prog_eps3.clear();
+	if (m->pub3) {
+		const CALModel3D* p = m->pub3;
+		for (int e = 0; e < p->num_of_elementary_processes; ++e)
+			if (g->index_of((const void*)p->elementary_processes[e]) >= 0)
+				m->prog_eps3.push_back(p->elementary_processes[e]);
+	} else {
+		const CALModel2D* p = &m->pub;
+		for (int e = 0; e < p->num_of_elementary_processes; ++e)
+			if (g->index_of((const void*)p->elementary_processes[e]) >= 0)
+				m->prog_eps.push_back(p->elementary_processes[e]);
+	}
 	return 0;
 }
 
-extern "C" int calhipBindGlobal2D(struct CALModel2D* ca2D, const char* name, const void* host_address, size_t bytes)
+int bind_global(ModelX* m, const char* name, const void* host_address, size_t bytes, const char* fn)
 {
-	if (!ca2D || !valid_identifier(name) || !host_address || bytes == 0) {
-		set_error(64, "calhipBindGlobal2D: give the global's C name, its host address and its size");
+	if (!m || !valid_identifier(name) || !host_address || bytes == 0) {
+		set_error(64, std::string(fn) + ": give the global's C name, its host address and its size");
 		return -1;
 	}
-	GenericProgram* g = generic_of(X(ca2D), true);
+	GenericProgram* g = generic_of(m, true);
 	if (!g) return -1;
 	for (auto& gl : g->globals)
 		if (gl.name == name) {
@@ -548,24 +742,24 @@ extern "C" int calhipBindGlobal2D(struct CALModel2D* ca2D, const char* name, con
 	return 0;
 }
 
-extern "C" int calhipCheckProcessSource2D(const char* source, const char* const* names, int n_names, char* log,
-                                          size_t log_bytes)
+int check_source(const char* source, const char* const* names, int n_names, char* log, size_t log_bytes, int dims,
+                 const char* fn)
 {
 	if (log && log_bytes) log[0] = '\0';
 	if (!source || n_names < 0 || (n_names > 0 && !names)) {
-		set_error(64, "calhipCheckProcessSource2D: missing argument");
+		set_error(64, std::string(fn) + ": missing argument");
 		return -1;
 	}
 	std::vector<std::string> nm;
 	for (int k = 0; k < n_names; ++k) {
 		if (!valid_identifier(names[k])) {
-			set_error(64, "calhipCheckProcessSource2D: process names must be C identifiers");
+			set_error(64, std::string(fn) + ": process names must be C identifiers");
 			return -1;
 		}
 		nm.push_back(names[k]);
 	}
 	std::vector<char> code;
-	if (compile(process_program_text(std::string(source) + "\n", nm), code)) {
+	if (compile(process_program_text(std::string(source) + "\n", nm, dims), dims, code)) {
 		if (log && log_bytes) {
 			const char* e = calhipGetErrorString();
 			strncpy(log, e ? e : "", log_bytes - 1);
@@ -574,4 +768,48 @@ extern "C" int calhipCheckProcessSource2D(const char* source, const char* const*
 		return -1;
 	}
 	return 0;
+}
+
+}  // namespace
+
+extern "C" int calhipAddProcessSource2D(struct CALModel2D* ca2D, const char* source)
+{
+	return add_source(ca2D ? X(ca2D) : nullptr, source, "calhipAddProcessSource2D");
+}
+
+extern "C" int calhipBindProcess2D(struct CALModel2D* ca2D, CALCallbackFunc2D process, const char* device_name)
+{
+	return bind_process(ca2D ? X(ca2D) : nullptr, (const void*)process, device_name, "calhipBindProcess2D");
+}
+
+extern "C" int calhipBindGlobal2D(struct CALModel2D* ca2D, const char* name, const void* host_address, size_t bytes)
+{
+	return bind_global(ca2D ? X(ca2D) : nullptr, name, host_address, bytes, "calhipBindGlobal2D");
+}
+
+extern "C" int calhipCheckProcessSource2D(const char* source, const char* const* names, int n_names, char* log,
+                                          size_t log_bytes)
+{
+	return check_source(source, names, n_names, log, log_bytes, 2, "calhipCheckProcessSource2D");
+}
+
+extern "C" int calhipAddProcessSource3D(struct CALModel3D* ca3D, const char* source)
+{
+	return add_source(ca3D ? &X3(ca3D)->core : nullptr, source, "calhipAddProcessSource3D");
+}
+
+extern "C" int calhipBindProcess3D(struct CALModel3D* ca3D, CALCallbackFunc3D process, const char* device_name)
+{
+	return bind_process(ca3D ? &X3(ca3D)->core : nullptr, (const void*)process, device_name, "calhipBindProcess3D");
+}
+
+extern "C" int calhipBindGlobal3D(struct CALModel3D* ca3D, const char* name, const void* host_address, size_t bytes)
+{
+	return bind_global(ca3D ? &X3(ca3D)->core : nullptr, name, host_address, bytes, "calhipBindGlobal3D");
+}
+
+extern "C" int calhipCheckProcessSource3D(const char* source, const char* const* names, int n_names, char* log,
+                                          size_t log_bytes)
+{
+	return check_source(source, names, n_names, log, log_bytes, 3, "calhipCheckProcessSource3D");
 }

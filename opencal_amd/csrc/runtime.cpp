@@ -3,11 +3,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "engine.hpp"
 
 namespace calhip {
 
-static int init_stream(ModelX* m)
+int ensure_stream(ModelX* m)
 {
 	if (m->stream) return 0;
 	int n = 0;
@@ -45,7 +47,7 @@ static int ensure_active_buffers(ModelX* m)
 int ensure_device(ModelX* m)
 {
 	if (m->phase == Phase::Device) return 0;
-	if (init_stream(m)) return -1;
+	if (ensure_stream(m)) return -1;
 	CALModel2D* p = &m->pub;
 	size_t n = m->cells();
 	for (SubX* s : m->subs) {
@@ -74,9 +76,18 @@ int ensure_device(ModelX* m)
 			return -1;
 		// the device list mirrors A.cells (linear indices); count 0 <=> NULL list
 		std::vector<int> lin;
-		int cnt = p->A.cells ? p->A.size_current : 0;
-		lin.resize((size_t)cnt + 1);
-		for (int k = 0; k < cnt; ++k) lin[k] = p->A.cells[k].i * p->columns + p->A.cells[k].j;
+		int cnt;
+		if (m->pub3) {
+			const CALModel3D* q = m->pub3;
+			cnt = q->A.cells ? q->A.size_current : 0;
+			lin.resize((size_t)cnt + 1);
+			for (int k = 0; k < cnt; ++k)
+				lin[k] = (q->A.cells[k].k * q->rows + q->A.cells[k].i) * q->columns + q->A.cells[k].j;
+		} else {
+			cnt = p->A.cells ? p->A.size_current : 0;
+			lin.resize((size_t)cnt + 1);
+			for (int k = 0; k < cnt; ++k) lin[k] = p->A.cells[k].i * p->columns + p->A.cells[k].j;
+		}
 		if (cnt && CALHIP_CHECK(hipMemcpyAsync(m->d_cells, lin.data(), sizeof(int) * (size_t)cnt,
 		                                       hipMemcpyHostToDevice, m->stream)))
 			return -1;
@@ -92,6 +103,9 @@ int ensure_device(ModelX* m)
 	if (CALHIP_CHECK(hipStreamSynchronize(m->stream))) return -1;
 	m->phase = Phase::Device;
 	m->stats.uploads++;
+	// the host mirrors hold what was just uploaded
+	device_changed(m);
+	for (SubX* s : m->subs) s->host_epoch = m->dev_epoch;
 	if (m->prog && m->prog->after_upload(m)) return -1;
 	return 0;
 }
@@ -141,14 +155,35 @@ int ensure_host(ModelX* m)
 		if (cnt && CALHIP_CHECK(hipMemcpy(lin.data(), m->d_cells, sizeof(int) * (size_t)cnt,
 		                                  hipMemcpyDeviceToHost)))
 			return -1;
-		free(p->A.cells);
-		p->A.cells = nullptr;
-		p->A.size_current = cnt;
-		if (cnt) {
-			p->A.cells = (CALCell2D*)malloc(sizeof(CALCell2D) * (size_t)cnt);
-			for (int k = 0; k < cnt; ++k) {
-				p->A.cells[k].i = lin[k] / p->columns;
-				p->A.cells[k].j = lin[k] % p->columns;
+		if (m->pub3) {
+			// OpenCAL's 3D list is in (i, j, k) order (cal3D.c:412-424)
+			CALModel3D* q = m->pub3;
+			free(q->A.cells);
+			q->A.cells = nullptr;
+			q->A.size_current = cnt;
+			if (cnt) {
+				q->A.cells = (CALCell3D*)malloc(sizeof(CALCell3D) * (size_t)cnt);
+				const long long plane = (long long)q->rows * q->columns;
+				for (int k = 0; k < cnt; ++k) {
+					const long long r = lin[k] % plane;
+					q->A.cells[k].k = (int)(lin[k] / plane);
+					q->A.cells[k].i = (int)(r / q->columns);
+					q->A.cells[k].j = (int)(r % q->columns);
+				}
+				std::sort(q->A.cells, q->A.cells + cnt, [](const CALCell3D& a, const CALCell3D& b) {
+					return a.i != b.i ? a.i < b.i : (a.j != b.j ? a.j < b.j : a.k < b.k);
+				});
+			}
+		} else {
+			free(p->A.cells);
+			p->A.cells = nullptr;
+			p->A.size_current = cnt;
+			if (cnt) {
+				p->A.cells = (CALCell2D*)malloc(sizeof(CALCell2D) * (size_t)cnt);
+				for (int k = 0; k < cnt; ++k) {
+					p->A.cells[k].i = lin[k] / p->columns;
+					p->A.cells[k].j = lin[k] % p->columns;
+				}
 			}
 		}
 		// flags changed since the last rebuild show up as a pending delta
@@ -164,6 +199,21 @@ int ensure_host(ModelX* m)
 		s->nxt_dirty = false;
 	}
 	m->phase = Phase::Host;
+	m->stats.downloads++;
+	return 0;
+}
+
+int host_read(ModelX* m, void* q)
+{
+	SubX* s = S(q);
+	if (m->phase == Phase::Host || s->host_only) return 0;
+	if (s->tstate != Transient::None) return ensure_host(m);   // logical content is not in the buffer
+	if (s->host_epoch == m->dev_epoch) return 0;
+	if (CALHIP_CHECK(hipMemcpyAsync(s->h_cur(), s->d_cur, m->cells() * (size_t)s->elem, hipMemcpyDeviceToHost,
+	                                m->stream)) ||
+	    CALHIP_CHECK(hipStreamSynchronize(m->stream)))
+		return -1;
+	s->host_epoch = m->dev_epoch;
 	m->stats.downloads++;
 	return 0;
 }
@@ -188,6 +238,7 @@ int materialise_next(ModelX* m, SubX* s)
 // calInitSubstate(Next)2D* on the device.
 int device_fill(ModelX* m, SubX* s, double v, bool both)
 {
+	device_changed(m);
 	if (s->tstate != Transient::None && m->prog) {
 		int r = m->prog->transient_init(m, s, v, both);
 		if (r != 0) return r < 0 ? -1 : 0;
@@ -214,6 +265,7 @@ int device_fill(ModelX* m, SubX* s, double v, bool both)
 int device_rebuild_active(ModelX* m)
 {
 	if (!m->active() || !m->flags_dirty) return 0;
+	device_changed(m);
 	if (k::rebuild_active(m)) return -1;
 	m->flags_dirty = false;
 	m->stats.rebuilds++;
@@ -223,6 +275,7 @@ int device_rebuild_active(ModelX* m)
 // calUpdate2D on the device: rebuild, then flip every written next layer.
 int device_update(ModelX* m)
 {
+	device_changed(m);
 	if (device_rebuild_active(m)) return -1;
 	for (SubX* s : m->subs)
 		if (!s->host_only && !s->single && s->nxt_dirty) flip(s);

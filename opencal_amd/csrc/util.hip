@@ -34,6 +34,23 @@ __global__ void fill_kernel(T* __restrict__ buf, T v, size_t n, const int* __res
 	}
 }
 
+// OpenCAL's update of one substate in active mode (calUpdateSubstate2D*,
+// cal2D.c:694-713): next -> current at the listed cells, or over the whole
+// grid when the list is empty (A.cells == NULL).
+template <typename T>
+__global__ void copy_cells_kernel(T* __restrict__ dst, const T* __restrict__ src, size_t n,
+                                  const int* __restrict__ cells, const int* __restrict__ count)
+{
+	size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	size_t stride = (size_t)gridDim.x * blockDim.x;
+	int cnt = count[0];
+	if (cnt > 0) {
+		for (size_t k = tid; k < (size_t)cnt; k += stride) dst[cells[k]] = src[cells[k]];
+	} else {
+		for (size_t c = tid; c < n; c += stride) dst[c] = src[c];
+	}
+}
+
 __device__ __forceinline__ int count_bytes16(const uint8_t* p, size_t base, size_t n)
 {
 	int c = 0;
@@ -141,6 +158,28 @@ int copy(ModelX* m, void* dst, const void* src, size_t bytes)
 {
 	m->stats.launches++;
 	return CALHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, m->stream));
+}
+
+int copy_active(ModelX* m, void* dst, const void* src, int elem)
+{
+	size_t n = m->cells();
+	int g = grid_for(n);
+	switch (elem) {
+	case 1:
+		hipLaunchKernelGGL(copy_cells_kernel<int8_t>, dim3(g), dim3(FILL_BLOCK), 0, m->stream, (int8_t*)dst,
+		                   (const int8_t*)src, n, m->d_cells, m->d_count);
+		break;
+	case 4:
+		hipLaunchKernelGGL(copy_cells_kernel<int32_t>, dim3(g), dim3(FILL_BLOCK), 0, m->stream, (int32_t*)dst,
+		                   (const int32_t*)src, n, m->d_cells, m->d_count);
+		break;
+	default:
+		hipLaunchKernelGGL(copy_cells_kernel<double>, dim3(g), dim3(FILL_BLOCK), 0, m->stream, (double*)dst,
+		                   (const double*)src, n, m->d_cells, m->d_count);
+		break;
+	}
+	m->stats.launches++;
+	return CALHIP_CHECK(hipGetLastError());
 }
 
 int rebuild_active(ModelX* m)

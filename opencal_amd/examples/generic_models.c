@@ -15,8 +15,11 @@
  *            adds its index weight to each neighbour's next value
  *            (calAddNextX2Di) and reads its own next value
  *            (calGetNext2Di) in a second process; any neighbourhood and
- *            boundary */
+ *            boundary
+ *   gred_*   a byte, an int and a real substate for the reductions of
+ *            cal2DReduction.h, on the host arrays and on the GPU */
 #include <cal2D.h>
+#include <cal2DReduction.h>
 #include <cal2DRun.h>
 #include <cal2DUnsafe.h>
 #include <calhip2D.h>
@@ -364,6 +367,78 @@ void ggift_destroy(void* handle)
 	free(s);
 }
 
+/* ------------------------------------------------------------ reductions */
+
+CALHIP_SOURCE(gred_source,
+struct gred_state {
+	struct CALSubstate2Db* b;
+	struct CALSubstate2Di* i;
+	struct CALSubstate2Dr* r;
+};
+CALHIP_GLOBAL struct gred_state gredS;
+
+void gred_keep(struct CALModel2D* ca, int i, int j)
+{
+	calSet2Db(ca, gredS.b, i, j, calGet2Db(ca, gredS.b, i, j));
+	calSet2Di(ca, gredS.i, i, j, calGet2Di(ca, gredS.i, i, j));
+	calSet2Dr(ca, gredS.r, i, j, calGet2Dr(ca, gredS.r, i, j));
+}
+)
+
+typedef struct {
+	struct CALModel2D* model;
+} gred_t;
+
+/* a model holding the three arrays in a byte, an int and a real substate */
+void* gred_create(int rows, int cols, const signed char* b, const int* iv, const double* r)
+{
+	gred_t* s = (gred_t*)calloc(1, sizeof(gred_t));
+	size_t n = (size_t)rows * cols;
+	if (!s) return NULL;
+	s->model = calCADef2D(rows, cols, CAL_VON_NEUMANN_NEIGHBORHOOD_2D, CAL_SPACE_TOROIDAL, CAL_NO_OPT);
+	if (!s->model) return NULL;
+	gredS.b = calAddSubstate2Db(s->model);
+	gredS.i = calAddSubstate2Di(s->model);
+	gredS.r = calAddSubstate2Dr(s->model);
+	memcpy(gredS.b->current, b, n);
+	memcpy(gredS.b->next, b, n);
+	memcpy(gredS.i->current, iv, n * sizeof(int));
+	memcpy(gredS.i->next, iv, n * sizeof(int));
+	memcpy(gredS.r->current, r, n * sizeof(double));
+	memcpy(gredS.r->next, r, n * sizeof(double));
+	calAddElementaryProcess2D(s->model, gred_keep);
+	if (calhipAddProcessSource2D(s->model, gred_source) || calhipBindProcess2D(s->model, gred_keep, "gred_keep") ||
+	    calhipBindGlobal2D(s->model, "gredS", &gredS, sizeof gredS))
+		return NULL;
+	return s;
+}
+
+/* calReductionOperation2D{b,i,r} (which = 0, 1, 2) as a double */
+double gred_reduce(void* handle, int which, int op)
+{
+	gred_t* s = (gred_t*)handle;
+	if (which == 0) return (double)calReductionOperation2Db(s->model, gredS.b, (enum REDUCTION_OPERATION)op);
+	if (which == 1) return (double)calReductionOperation2Di(s->model, gredS.i, (enum REDUCTION_OPERATION)op);
+	return calReductionOperation2Dr(s->model, gredS.r, (enum REDUCTION_OPERATION)op);
+}
+
+/* a transition that keeps every value: the model moves to the GPU */
+int gred_step(void* handle)
+{
+	calGlobalTransitionFunction2D(((gred_t*)handle)->model);
+	return calhipGetLastError();
+}
+
+int gred_on_device(void* handle) { return calhipIsOnDevice2D(((gred_t*)handle)->model); }
+
+void gred_destroy(void* handle)
+{
+	gred_t* s = (gred_t*)handle;
+	if (!s) return;
+	calFinalize2D(s->model);
+	free(s);
+}
+
 /* Compile every source above for the GPU without running it (0 when all
  * compile; otherwise the compiler's log). */
 int generic_check_sources(char* log, size_t log_bytes)
@@ -373,6 +448,8 @@ int generic_check_sources(char* log, size_t log_bytes)
 	const char* gift[] = {"ggift_give", "ggift_look"};
 	if (calhipCheckProcessSource2D(gsct_source, sct, 3, log, log_bytes)) return 1;
 	if (calhipCheckProcessSource2D(glife_source, life, 1, log, log_bytes)) return 2;
+	const char* red[] = {"gred_keep"};
 	if (calhipCheckProcessSource2D(ggift_source, gift, 2, log, log_bytes)) return 3;
+	if (calhipCheckProcessSource2D(gred_source, red, 1, log, log_bytes)) return 4;
 	return 0;
 }

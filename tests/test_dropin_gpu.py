@@ -39,8 +39,8 @@ BIN_G = os.path.join(native.REF_DIR, "dropin_sciddicaT_generic")
 @pytest.mark.skipif(not os.path.exists(BIN_G), reason="dropin_sciddicaT_generic not built (needs the reference tree)")
 def test_reference_program_on_the_generic_device_program(tmp_path):
     """The same program with its own elementary processes compiled for the
-    GPU at run time (its globals marked CALHIP_GLOBAL, five bind calls): the
-    shipped width_final.txt byte for byte."""
+    GPU at run time (the file's own text, five bind calls): the shipped
+    width_final.txt byte for byte."""
     print("case dropin sciddicaT generic 610x496 4000 steps", flush=True)
     data = tmp_path / "data"
     data.mkdir()
@@ -80,3 +80,67 @@ def test_reference_sciara_program_on_the_engine(tmp_path):
         assert text == ["%.9f" % v for v in port[k].ravel()], name
         got = np.array(text, dtype=np.float64)
         assert np.abs(got - libm[k].ravel()).max() <= 1e-9 * scale + 1e-9, name
+
+
+BIN_MB = os.path.join(native.REF_DIR, "dropin_mbusu")
+BIN_MB_REF = os.path.join(native.REF_DIR, "calomp_mbusu")
+MB_NAMES = ["teta", "moist_cont", "psi", "k", "h", "dqdh", "convergence", "moist_diff"]
+
+
+def _floats(text):
+    return [float(t) for t in text.replace(",", " ").split()]
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_MB), reason="dropin_mbusu not built (needs the reference tree)")
+def test_reference_mbusu_3d_on_the_generic_device_program(tmp_path):
+    """The reference 3D example mbusu (OpenCAL-OMP/examples/calomp_mbusu-glut,
+    160x30x130 cells, 8 substates, pow-heavy Richards-equation process),
+    unchanged, its process compiled for the GPU at run time, run to its own
+    stop condition (ten simulated days, 8662 steps).  Its outputs against the
+    files the reference ships from its OpenMP build: deltat.txt line for line
+    and ris10g.txt (the moisture section, "%f") within 2e-6 — the reference's
+    own serial and OpenMP outputs differ by up to 2.4e-5; the GPU's pow
+    differs from glibc's by an ulp."""
+    print("case dropin mbusu 160x30x130 to the stop condition", flush=True)
+    r = subprocess.run([BIN_MB], cwd=tmp_path, capture_output=True, text=True, timeout=400)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "steps 8662" in r.stdout, r.stdout + r.stderr
+    want_dt = gzip.open(os.path.join(GOLDEN, "mbusu_deltat-orig-opencal-omp.txt.gz"), "rt").read().splitlines()
+    got_dt = (tmp_path / "deltat.txt").read_text().splitlines()
+    assert len(got_dt) == 8662 and got_dt == want_dt[:8662]
+    want = _floats(gzip.open(os.path.join(GOLDEN, "mbusu_ris10g-orig-opencal-omp.txt.gz"), "rt").read())
+    got = _floats((tmp_path / "ris10g.txt").read_text())
+    assert len(got) == len(want) == 130 * 160
+    diff = max(abs(a - b) for a, b in zip(got, want))
+    same = sum(1 for a, b in zip(got, want) if a == b)
+    print(f"mbusu ris10g: max |diff| {diff:.2e}, {same}/{len(got)} values printed identically", flush=True)
+    assert diff <= 2e-6
+
+
+@pytest.mark.skipif(not (os.path.exists(BIN_MB) and os.path.exists(BIN_MB_REF)),
+                    reason="mbusu builds not present (needs the reference tree)")
+def test_reference_mbusu_3d_state_against_the_reference_engine(tmp_path):
+    """All eight substates of mbusu after 25 steps: the GPU run against the
+    reference OpenCAL-OMP 3D engine running the same file on the CPU.  The
+    largest difference stays below 1e-12 of the substate's largest value
+    (the GPU's pow against glibc's; moist_diff is a difference of two close
+    values, so its cell-wise relative error is not meaningful)."""
+    import numpy as np
+    print("case dropin mbusu 25 steps vs reference engine", flush=True)
+    (tmp_path / "gpu").mkdir()
+    (tmp_path / "cpu").mkdir()
+    r = subprocess.run([BIN_MB, "25", str(tmp_path / "gpu.bin")], cwd=tmp_path / "gpu", capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = subprocess.run([BIN_MB_REF, "25", str(tmp_path / "cpu.bin")], cwd=tmp_path / "cpu", capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    g = np.fromfile(tmp_path / "gpu.bin", np.float64).reshape(8, -1)
+    c = np.fromfile(tmp_path / "cpu.bin", np.float64).reshape(8, -1)
+    assert g.shape == c.shape == (8, 160 * 30 * 130)
+    assert (tmp_path / "gpu" / "deltat.txt").read_text() == (tmp_path / "cpu" / "deltat.txt").read_text()
+    for s, name in enumerate(MB_NAMES):
+        rel = float(np.max(np.abs(g[s] - c[s])) / np.max(np.abs(c[s])))
+        exact = int(np.sum(g[s] == c[s]))
+        print(f"mbusu {name}: max |diff| / max |value| {rel:.2e}, {exact}/{g.shape[1]} bit-identical", flush=True)
+        assert rel <= 1e-12, name
