@@ -96,25 +96,34 @@ def test_reference_mbusu_3d_on_the_generic_device_program(tmp_path):
     """The reference 3D example mbusu (OpenCAL-OMP/examples/calomp_mbusu-glut,
     160x30x130 cells, 8 substates, pow-heavy Richards-equation process),
     unchanged, its process compiled for the GPU at run time, run to its own
-    stop condition (ten simulated days, 8662 steps).  Its outputs against the
-    files the reference ships from its OpenMP build: deltat.txt line for line
-    and ris10g.txt (the moisture section, "%f") within 2e-6 — the reference's
-    own serial and OpenMP outputs differ by up to 2.4e-5; the GPU's pow
-    differs from glibc's by an ulp."""
+    stop condition (ten simulated days, 8662 steps).  deltat.txt equals the
+    shipped file line for line.  ris10g.txt (the moisture section, "%f")
+    against the reference built from its sources here
+    (make_mbusu_fixture.py): within one unit of the last printed digit (the
+    GPU's pow differs from glibc's by an ulp); against the file the reference
+    ships from its OpenMP build: within 4e-5, as the reference's own build
+    here (3.3e-5; its shipped serial and OpenMP files differ by 2.4e-5)."""
+    import time
     print("case dropin mbusu 160x30x130 to the stop condition", flush=True)
+    t0 = time.perf_counter()
     r = subprocess.run([BIN_MB], cwd=tmp_path, capture_output=True, text=True, timeout=400)
+    wall = time.perf_counter() - t0
     assert r.returncode == 0, r.stdout + r.stderr
+    print(f"mbusu: 8662 steps in {wall:.1f} s of wall time ({1e3 * wall / 8662:.2f} ms/step, "
+          f"process start and compilation included)", flush=True)
     assert "steps 8662" in r.stdout, r.stdout + r.stderr
     want_dt = gzip.open(os.path.join(GOLDEN, "mbusu_deltat-orig-opencal-omp.txt.gz"), "rt").read().splitlines()
     got_dt = (tmp_path / "deltat.txt").read_text().splitlines()
     assert len(got_dt) == 8662 and got_dt == want_dt[:8662]
-    want = _floats(gzip.open(os.path.join(GOLDEN, "mbusu_ris10g-orig-opencal-omp.txt.gz"), "rt").read())
     got = _floats((tmp_path / "ris10g.txt").read_text())
-    assert len(got) == len(want) == 130 * 160
-    diff = max(abs(a - b) for a, b in zip(got, want))
-    same = sum(1 for a, b in zip(got, want) if a == b)
-    print(f"mbusu ris10g: max |diff| {diff:.2e}, {same}/{len(got)} values printed identically", flush=True)
-    assert diff <= 2e-6
+    for fixture, tol in (("mbusu_ris10g_reference_build.txt.gz", 1.01e-6),
+                         ("mbusu_ris10g-orig-opencal-omp.txt.gz", 4e-5)):
+        want = _floats(gzip.open(os.path.join(GOLDEN, fixture), "rt").read())
+        assert len(got) == len(want) == 130 * 160
+        diff = max(abs(a - b) for a, b in zip(got, want))
+        same = sum(1 for a, b in zip(got, want) if a == b)
+        print(f"mbusu ris10g vs {fixture}: max |diff| {diff:.2e}, {same}/{len(got)} printed identically", flush=True)
+        assert diff <= tol, fixture
 
 
 @pytest.mark.skipif(not (os.path.exists(BIN_MB) and os.path.exists(BIN_MB_REF)),
