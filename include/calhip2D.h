@@ -38,6 +38,18 @@ int calhipSynchronize2D(struct CALModel2D* ca2D);
 /* 1 if the model's state currently lives on the GPU. */
 int calhipIsOnDevice2D(struct CALModel2D* ca2D);
 
+/* The step loop of calRun2D (cal2DRun.c:129-136) for n steps starting at
+ * simulation->step: each step as calRunCAStep2D does it; afterwards
+ * simulation->step is the number of the next step (the stopping step when
+ * the stop condition held).  When no host callback runs between steps (no
+ * user global transition, no stop condition, a steering that is absent or
+ * has a device twin) the device program may batch the steps.  The model
+ * stays on the GPU.  Returns the number of steps run, -1 on error.  Has no
+ * counterpart in OpenCAL: it is calRun2D without its init, finalize and
+ * read-back, for callers that keep a model resident between runs. */
+struct CALRun2D;
+int calhipRunSteps2D(struct CALRun2D* simulation, int n);
+
 /* Counters since model creation. */
 struct calhipStats {
 	long long steps;           /* device program steps */
@@ -112,6 +124,15 @@ int calhipBindSciddicaT2D(struct CALModel2D* ca2D,
                           struct CALSubstate2Dr* z, struct CALSubstate2Dr* h,
                           struct CALSubstate2Dr* f[4],
                           const CALreal* p_r, const CALreal* p_epsilon);
+
+/* Declare the model's steering callback to be the reference SciddicaT
+ * steering (sciddicaT.c:133-146: every flow substate set to 0 with
+ * calInitSubstate2Dr; the active-cells variant, cal_sciddicaT-activecells-glut
+ * sciddicaT.c:137-144, with calInitSubstateNext2Dr).  The run loop then runs
+ * its device twin instead of the callback, and calRun2D / calhipRunSteps2D
+ * may advance the full-grid model two steps per sweep (the flows are never
+ * stored).  The callback must do exactly that reset. */
+int calhipBindSciddicaTSteering2D(struct CALModel2D* ca2D, void (*steering)(struct CALModel2D*));
 
 /* Attach the Life device program (one process, Moore, toroidal) to a byte or
  * int substate.  Exactly one of Qb / Qi is non-NULL. */

@@ -72,6 +72,8 @@ def _declare(L):
     L.calhipSynchronize2D.argtypes = [vp]
     L.calhipSyncHost2D.argtypes = [vp]
     L.calhipIsOnDevice2D.argtypes = [vp]
+    L.calhipRunSteps2D.argtypes = [vp, C.c_int]
+    L.calhipBindSciddicaTSteering2D.argtypes = [vp, vp]
     L.calhipSciddicaTBatchRun.argtypes = [C.c_int, ip, ip, dp, dp, C.POINTER(dp), C.POINTER(dp), C.c_int,
                                           C.POINTER(dp), dp]
     L.calhipSciddicaTForwardDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
@@ -89,6 +91,7 @@ def _declare_examples(E):
     for name in ("sciddicat_run", "sciddicat_init", "sciddicat_sync"):
         getattr(E, name).argtypes = [vp]
     E.sciddicat_steps.argtypes = [vp, C.c_int]
+    E.sciddicat_run_steps.argtypes = [vp, C.c_int]
     E.sciddicat_read.argtypes = [vp, dp, dp, ip]
     E.sciddicat_read_flow.argtypes = [vp, C.c_int, dp]
     E.sciddicat_model.restype = vp

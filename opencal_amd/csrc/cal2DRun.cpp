@@ -87,30 +87,50 @@ extern "C" void calRunFinalizeSimulation2D(struct CALRun2D* r)
 	}
 }
 
-// cal2DRun.c:125-139.  When no host callback runs between steps (no user
-// global transition, steering or stop condition) the remaining steps of a
-// bounded run go to the device program in one call, which may batch them
-// into fewer launches.  The model's state is in the host mirrors on return.
+// The step loop of calRun2D (cal2DRun.c:129-136) for n steps from r->step,
+// which ends on the number after the last step run.  When no host callback
+// runs between steps (no user global transition, no stop condition, and a
+// steering that is either absent or has a device twin) the steps after the
+// first go to the device program in one call, which may batch them into
+// fewer launches (e.g. two SciddicaT steps per sweep).
+extern "C" int calhipRunSteps2D(struct CALRun2D* r, int n)
+{
+	ModelX* x = X(r->ca2D);
+	const bool twin = r->steering && x->prog && x->prog_steering == r->steering;
+	const bool batchable = !r->globalTransition && !r->stopCondition && x->prog &&
+	                       (!r->steering || (twin && r->UPDATE_MODE == CAL_UPDATE_IMPLICIT));
+	int done = 0;
+	while (done < n) {
+		if (batchable && done > 0 && x->phase == Phase::Device) {
+			const int rest = n - done;
+			device_changed(x);
+			if (x->prog->run_steps(x, rest, twin)) return -1;
+			x->stats.steps += rest;
+			r->step += rest;
+			return n;
+		}
+		const CALbyte again = calRunCAStep2D(r);
+		if (calhipGetLastError()) return -1;
+		done++;
+		if (!again) return done;   // the reference leaves the step number at the stopping step
+		r->step++;
+	}
+	return done;
+}
+
+// cal2DRun.c:125-139.  The model's state is in the host mirrors on return.
 extern "C" void calRun2D(struct CALRun2D* r)
 {
 	calRunInitSimulation2D(r);
 	ModelX* x = X(r->ca2D);
-	bool batchable = !r->globalTransition && !r->steering && !r->stopCondition &&
-	                 r->final_step != CAL_RUN_LOOP && x->prog;
-	if (batchable && r->final_step >= r->initial_step) {
-		int n = r->final_step - r->initial_step + 1;
-		r->step = r->initial_step;
-		calGlobalTransitionFunction2D(r->ca2D);   // binds + uploads, one step
-		if (n > 1 && calhipGetLastError() == 0) {
-			device_changed(x);
-			if (x->prog->run_steps(x, n - 1) == 0) x->stats.steps += n - 1;
-		}
-		r->step = r->final_step + 1;
-	} else {
-		for (r->step = r->initial_step; r->step <= r->final_step || r->final_step == CAL_RUN_LOOP; r->step++) {
+	r->step = r->initial_step;
+	if (r->final_step == CAL_RUN_LOOP) {
+		for (;; r->step++) {
 			if (!calRunCAStep2D(r)) break;
 			if (calhipGetLastError()) break;
 		}
+	} else if (r->final_step >= r->initial_step) {
+		calhipRunSteps2D(r, r->final_step - r->initial_step + 1);
 	}
 	calRunFinalizeSimulation2D(r);
 	ensure_host(x);

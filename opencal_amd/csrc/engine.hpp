@@ -57,12 +57,18 @@ struct Program {
 	virtual const char* name() const = 0;
 	// calGlobalTransitionFunction2D on the device.
 	virtual int step(ModelX* m) = 0;
-	// Several steps when no host callback runs in between (default: loop).
-	virtual int run_steps(ModelX* m, int n) {
-		for (int k = 0; k < n; ++k)
-			if (int e = step(m)) return e;
+	// n steps with no host callback in between; with_steering: each step is
+	// followed by the steering twin and the update (calRunCAStep2D with a
+	// twin steering, implicit update).  Default: that loop.
+	virtual int run_steps(ModelX* m, int n, bool with_steering) {
+		for (int k = 0; k < n; ++k) {
+			if (int e = with_steering ? step_and_steer(m) : step(m)) return e;
+			if (with_steering && device_update_hook(m)) return -1;
+		}
 		return 0;
 	}
+	// calUpdate2D on the device (runtime.cpp's device_update)
+	static int device_update_hook(ModelX* m);
 	// calInitSubstate(Next)2Dr on a substate the program marked transient.
 	// Returns 1 when handled, 0 to fall back to an ordinary device fill (after
 	// materialising), <0 on error.

@@ -221,8 +221,9 @@ struct LifeProgram : Program {
 
 	bool tiles_fit(ModelX* m) const { return m->pub.columns % 64 == 0; }
 
-	int run_steps(ModelX* m, int n) override
+	int run_steps(ModelX* m, int n, bool with_steering) override
 	{
+		if (with_steering) return Program::run_steps(m, n, true);   // Life has no steering twin
 		if (q->nxt_dirty) flip(q);
 		if (n <= 1 || !tiles_fit(m) || getenv("CALHIP_LIFE_NO_TILES")) {
 			for (int k = 0; k < n; ++k)

@@ -272,6 +272,8 @@ int device_rebuild_active(ModelX* m)
 	return 0;
 }
 
+int Program::device_update_hook(ModelX* m) { return device_update(m); }
+
 // calUpdate2D on the device: rebuild, then flip every written next layer.
 int device_update(ModelX* m)
 {

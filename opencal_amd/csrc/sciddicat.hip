@@ -38,6 +38,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <stdlib.h>
+
 #include "engine.hpp"
 #include "device_common.hpp"
 #include "sciddicat_device.hpp"
@@ -318,6 +320,8 @@ struct SciddicaTProgram : Program {
 	double* d_trash = nullptr;   // sink for the stores of halo lanes
 	SweepShape shape = {0, 0};   // chunk length and prefetch depth (0: not chosen yet)
 	WetOrder wet_order;          // wet-first tile order of the fused sweep
+	int rpw2 = 0;                // chunk length of the two-step sweep (0: not chosen yet)
+	WetOrder wet_order2;         // its wet-first order
 
 	// flag-driven active step: set sizes [0..1] (current / next by parity)
 	// and receiver counts [2..3]; the row-major list is rebuilt on demand
@@ -504,6 +508,47 @@ struct SciddicaTProgram : Program {
 		return active ? step_active(m) : step_full(m);
 	}
 
+	// Device twin of the reference steering (sciddicaT.c:133-146; the active
+	// variant resets the next layer only): the flows become logically zero.
+	int steer(ModelX* m) override
+	{
+		for (int q = 0; q < 4; ++q)
+			if (device_fill(m, f[q], 0.0, !active)) return -1;   // calInitSubstate(Next)2Dr(f[q], 0)
+		return 0;
+	}
+
+	// Full grid with the steering twin: pairs of steps go to the two-step
+	// sweep (the steering's reset between them is what makes a step a
+	// function of z and h alone); the rest, one step at a time.
+	int run_steps(ModelX* m, int n, bool with_steering) override
+	{
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		if (!with_steering || active || !sweep2_supported(rows, cols) || getenv("CALHIP_SCIDDICAT_NO_2X"))
+			return Program::run_steps(m, n, with_steering);
+		while (n >= 2) {
+			if (check_entry(m)) return -1;
+			bool zero = true;
+			for (int q = 0; q < 4; ++q) zero = zero && f[q]->tstate == Transient::Zero;
+			if (!zero) break;
+			const double r = *p_r, eps = *p_eps;
+			if (!d_trash && CALHIP_CHECK(hipMalloc(&d_trash, 128 * sizeof(double)))) return -1;
+			if (rpw2 == 0)
+				rpw2 = sweep2_rows(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
+				                   (double*)h->d_nxt, d_trash, r, eps);
+			hipEvent_t ev;
+			timer_begin(m, &ev);
+			launch_sweep2(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
+			              (double*)h->d_nxt, d_trash, r, eps, rpw2, &wet_order2);
+			timer_end(m, ev);
+			m->stats.launches++;
+			if (CALHIP_CHECK(hipGetLastError())) return -1;
+			flip(h);
+			h_prev_valid = false;   // the flows stay zero: nothing to recover
+			n -= 2;
+		}
+		return Program::run_steps(m, n, true);
+	}
+
 	int transient_init(ModelX* m, SubX* s, double v, bool both) override
 	{
 		(void)m;
@@ -662,5 +707,17 @@ extern "C" int calhipBindSciddicaT2D(struct CALModel2D* ca2D, CALCallbackFunc2D 
 	m->prog_eps.push_back(flows);
 	m->prog_eps.push_back(width);
 	if (remove) m->prog_eps.push_back(remove);
+	return 0;
+}
+
+extern "C" int calhipBindSciddicaTSteering2D(struct CALModel2D* ca2D, void (*steering)(struct CALModel2D*))
+{
+	ModelX* m = X(ca2D);
+	if (!steering || !dynamic_cast<SciddicaTProgram*>(m->prog)) {
+		set_error(24, "calhipBindSciddicaTSteering2D: bind the SciddicaT program (calhipBindSciddicaT2D) first "
+		              "and give its steering callback");
+		return -1;
+	}
+	m->prog_steering = steering;
 	return 0;
 }

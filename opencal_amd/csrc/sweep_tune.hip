@@ -19,7 +19,7 @@
 
 #include "engine.hpp"
 #include "device_common.hpp"
-#include "sciddicat_device.hpp"
+#include "sciddicat_2step.hpp"
 
 namespace calhip {
 namespace sct {
@@ -116,6 +116,22 @@ void launch_sweep(hipStream_t st, int rows, int cols, const double* z, const dou
 		                   cols, g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps, ws);
 }
 
+bool sweep2_supported(int rows, int cols) { return cols % 2 == 0 && cols >= 8 && rows >= 8; }
+
+void launch_sweep2(hipStream_t st, int rows, int cols, const double* z, const double* h, double* hn, double* trash,
+                   double r, double eps, int rpw, WetOrder* wo)
+{
+	const SweepGeom g = sweep_geom(rows, cols, rpw);
+	WetSets ws = {};
+	int blocks = g.blocks;
+	if (wo && wo->next(st, g.n_logical, ws) == 0)
+		blocks = 8 * (g.per_xcd + ws.cap);
+	else
+		ws = WetSets{};
+	hipLaunchKernelGGL(step_fused_2x, dim3(blocks), dim3(64 * FUSED_WAVES), 0, st, z, h, hn, trash, rows, cols,
+	                   g.n_strips, g.n_logical, g.per_xcd, g.rpw, r, eps, ws);
+}
+
 // --------------------------------------------------------------- tuner
 
 namespace {
@@ -165,6 +181,47 @@ SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, cons
 	if (e1) (void)hipEventDestroy(e1);
 	(void)hipGetLastError();
 	tuned[key] = best;
+	return best;
+}
+
+namespace {
+std::map<std::pair<int, int>, int> tuned2;
+}  // namespace
+
+int sweep2_rows(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch, double* trash,
+                double r, double eps)
+{
+	if (const char* e = getenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X")) {
+		const int v = atoi(e);
+		if (v > 0) return v;
+	}
+	std::lock_guard<std::mutex> lk(tune_mu);
+	const auto key = std::make_pair(rows, cols);
+	auto it = tuned2.find(key);
+	if (it != tuned2.end()) return it->second;
+	static const int cand[] = {16, 24, 32, 48, 64};
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int best = 32;
+	float best_ms = 0.f;
+	if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+		for (int c : cand) {
+			WetOrder wo;
+			launch_sweep2(st, rows, cols, z, h, scratch, trash, r, eps, c, &wo);
+			(void)hipEventRecord(e0, st);
+			for (int k = 0; k < 3; ++k) launch_sweep2(st, rows, cols, z, h, scratch, trash, r, eps, c, &wo);
+			(void)hipEventRecord(e1, st);
+			float ms = 0.f;
+			if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+			if (best_ms == 0.f || ms < best_ms) {
+				best_ms = ms;
+				best = c;
+			}
+		}
+	}
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	(void)hipGetLastError();
+	tuned2[key] = best;
 	return best;
 }
 

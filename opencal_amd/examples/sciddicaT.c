@@ -155,7 +155,8 @@ void* sciddicat_create(int rows, int cols, const double* z, const double* h, dou
 	if (active) calRunAddStopConditionFunc2D(s->run, sciddicaT_stop);
 	if (calhipBindSciddicaT2D(s->model, sciddicaT_flows_computation, sciddicaT_width_update,
 	                          active ? sciddicaT_remove_inactive_cells : NULL, s->Q.z, s->Q.h, s->Q.f, &s->P.r,
-	                          &s->P.epsilon))
+	                          &s->P.epsilon) ||
+	    calhipBindSciddicaTSteering2D(s->model, sciddicaT_steering))
 		return NULL;
 	return s;
 }
@@ -199,6 +200,17 @@ int sciddicat_steps(void* handle, int n)
 		s->run->step++;
 		calRunCAStep2D(s->run);
 	}
+	return calhipGetLastError();
+}
+
+/* n steps of calRun2D's loop (calhipRunSteps2D): steps may be batched. */
+int sciddicat_run_steps(void* handle, int n)
+{
+	sciddicat_t* s = (sciddicat_t*)handle;
+	S = s;
+	s->run->step++;
+	if (calhipRunSteps2D(s->run, n) < 0) return -1;
+	s->run->step--;   /* sciddicat_steps numbers a step before running it */
 	return calhipGetLastError();
 }
 

@@ -49,6 +49,11 @@ class SciddicaT:
     def steps(self, n):
         check(self.E.sciddicat_steps(self.handle, int(n)), "sciddicat_steps")
 
+    def run_steps(self, n):
+        """n steps of calRun2D's loop (calhipRunSteps2D): the full-grid model
+        with its steering twin advances two steps per sweep."""
+        check(self.E.sciddicat_run_steps(self.handle, int(n)), "sciddicat_run_steps")
+
     def sync(self):
         check(self.E.sciddicat_sync(self.handle), "sciddicat_sync")
 

@@ -33,6 +33,27 @@ def test_reference_program_runs_unchanged_on_the_engine(tmp_path):
     assert (data / "width_final.txt").read_text() == want
 
 
+BIN_2X = os.path.join(native.REF_DIR, "dropin_sciddicaT_2x")
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_2X), reason="dropin_sciddicaT_2x not built (needs the reference tree)")
+def test_reference_program_two_steps_per_sweep(tmp_path):
+    """The reference program with a third added line (its steering declared
+    as the flow reset): calRun2D advances two steps per sweep and the shipped
+    width_final.txt is reproduced byte for byte."""
+    print("case dropin sciddicaT 2x 610x496 4000 steps", flush=True)
+    data = tmp_path / "data"
+    data.mkdir()
+    for name, src in (("dem.txt", "sciddicaT_610x496_dem.txt.gz"), ("source.txt", "sciddicaT_610x496_source.txt.gz")):
+        with gzip.open(os.path.join(GOLDEN, src), "rb") as f, open(data / name, "wb") as g:
+            shutil.copyfileobj(f, g)
+    r = subprocess.run([BIN_2X], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Simulation terminated" in r.stdout
+    want = gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
+    assert (data / "width_final.txt").read_text() == want
+
+
 BIN_G = os.path.join(native.REF_DIR, "dropin_sciddicaT_generic")
 
 
