@@ -320,7 +320,7 @@ struct SciddicaTProgram : Program {
 	double* d_trash = nullptr;   // sink for the stores of halo lanes
 	SweepShape shape = {0, 0};   // chunk length and prefetch depth (0: not chosen yet)
 	WetOrder wet_order;          // wet-first tile order of the fused sweep
-	int rpw2 = 0;                // chunk length of the two-step sweep (0: not chosen yet)
+	Sweep2Shape shape2 = {0, 0}; // tiles of the two-step sweep (0: not chosen yet)
 	WetOrder wet_order2;         // its wet-first order
 
 	// flag-driven active step: set sizes [0..1] (current / next by parity)
@@ -532,13 +532,13 @@ struct SciddicaTProgram : Program {
 			if (!zero) break;
 			const double r = *p_r, eps = *p_eps;
 			if (!d_trash && CALHIP_CHECK(hipMalloc(&d_trash, 128 * sizeof(double)))) return -1;
-			if (rpw2 == 0)
-				rpw2 = sweep2_rows(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
-				                   (double*)h->d_nxt, d_trash, r, eps);
+			if (shape2.s_rows == 0)
+				shape2 = sweep2_shape(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
+				                      (double*)h->d_nxt, d_trash, r, eps);
 			hipEvent_t ev;
 			timer_begin(m, &ev);
 			launch_sweep2(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
-			              (double*)h->d_nxt, d_trash, r, eps, rpw2, &wet_order2);
+			              (double*)h->d_nxt, d_trash, r, eps, shape2, &wet_order2);
 			timer_end(m, ev);
 			m->stats.launches++;
 			if (CALHIP_CHECK(hipGetLastError())) return -1;

@@ -609,13 +609,16 @@ SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, cons
                        double* trash, double r, double eps);
 // Two steps per sweep (sciddicat_2step.hpp, flows reset between the steps):
 // the grids it supports (even cols >= 8, rows >= 8), one launch z, h -> hn,
-// and its row-chunk length (CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X, else tuned
-// on first use like sweep_shape).
+// and its tile shape (CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X = small-tile rows,
+// CALHIP_SCIDDICAT_2X_GROUP = small tiles per big tile, else 16 and 2).
+struct Sweep2Shape {
+	int s_rows, bigk;   // small-tile rows, small tiles per big tile
+};
 bool sweep2_supported(int rows, int cols);
 void launch_sweep2(hipStream_t st, int rows, int cols, const double* z, const double* h, double* hn, double* trash,
-                   double r, double eps, int rpw, WetOrder* wo);
-int sweep2_rows(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch, double* trash,
-                double r, double eps);
+                   double r, double eps, Sweep2Shape shape, WetOrder* wo);
+Sweep2Shape sweep2_shape(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch,
+                         double* trash, double r, double eps);
 
 }  // namespace sct
 }  // namespace calhip

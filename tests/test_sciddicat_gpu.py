@@ -151,18 +151,21 @@ def test_mass_conservation_4096():
     m.close()
 
 
-@pytest.mark.parametrize("rows,cols,rpw", [(8, 8, 16), (9, 10, 16), (64, 128, 16), (70, 260, 24), (130, 384, 48),
-                                           (41, 136, 64), (300, 200, 32)])
-def test_two_step_sweep(rows, cols, rpw, monkeypatch):
+@pytest.mark.parametrize("rows,cols,s_rows,group", [(8, 8, 16, 4), (9, 10, 16, 1), (64, 128, 16, 4), (70, 260, 24, 3),
+                                                    (130, 384, 48, 1), (41, 136, 8, 8), (300, 200, 16, 2),
+                                                    (257, 512, 16, 4)])
+def test_two_step_sweep(rows, cols, s_rows, group, monkeypatch):
     """calRun2D / calhipRunSteps2D with the steering twin advance the full
-    grid two steps per sweep (h1 kept in registers): bit-exact against the
-    oracle for even and odd step counts, chunk lengths above and below the
-    grid height, grids of 8 columns (every halo column wraps) and strips of
-    2..128 columns, wet cells on every strip edge, and when mixed with
-    single steps."""
+    grid two steps per sweep (h1 kept in registers), in big tiles of `group`
+    small tiles of `s_rows` rows with the small tiles that met wet cells run
+    first on their own: bit-exact against the oracle for even and odd step
+    counts, tiles taller than the grid, grids of 8 columns (every halo column
+    wraps) and strips of 2..128 columns, wet cells on every strip edge, and
+    when mixed with single steps."""
     from opencal_amd.models import SciddicaT
-    print(f"case two-step rows={rows} cols={cols} rpw={rpw}", flush=True)
-    monkeypatch.setenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X", str(rpw))
+    print(f"case two-step rows={rows} cols={cols} s_rows={s_rows} group={group}", flush=True)
+    monkeypatch.setenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X", str(s_rows))
+    monkeypatch.setenv("CALHIP_SCIDDICAT_2X_GROUP", str(group))
     z, h, _ = synth.sciddicat_inputs(rows, cols, rows + cols, patch=min(8, rows // 2))
     s = synth.Stream(rows * 7 + cols)
     for j in list(range(0, cols, 128)) + [cols - 1]:
