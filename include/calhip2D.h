@@ -134,6 +134,13 @@ int calhipBindSciddicaT2D(struct CALModel2D* ca2D,
  * stored).  The callback must do exactly that reset. */
 int calhipBindSciddicaTSteering2D(struct CALModel2D* ca2D, void (*steering)(struct CALModel2D*));
 
+/* Device check of the SciddicaT kernels' division of the level sum by the
+ * count of remaining levels (sciddicaT.c:72-73), which they compute with a
+ * multiply and two FMAs instead of the IEEE division sequence: for n seeded
+ * doubles and every count 0..5, the number of results whose bits differ from
+ * the IEEE quotient (0 expected), or -1 on error. */
+long long calhipSciddicaTCheckDivision(long long n, unsigned long long seed);
+
 /* Attach the Life device program (one process, Moore, toroidal) to a byte or
  * int substate.  Exactly one of Qb / Qi is non-NULL. */
 int calhipBindLife2D(struct CALModel2D* ca2D, CALCallbackFunc2D transition,

@@ -74,6 +74,8 @@ def _declare(L):
     L.calhipIsOnDevice2D.argtypes = [vp]
     L.calhipRunSteps2D.argtypes = [vp, C.c_int]
     L.calhipBindSciddicaTSteering2D.argtypes = [vp, vp]
+    L.calhipSciddicaTCheckDivision.restype = C.c_longlong
+    L.calhipSciddicaTCheckDivision.argtypes = [C.c_longlong, C.c_ulonglong]
     L.calhipSciddicaTBatchRun.argtypes = [C.c_int, ip, ip, dp, dp, C.POINTER(dp), C.POINTER(dp), C.c_int,
                                           C.POINTER(dp), dp]
     L.calhipSciddicaTForwardDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,

@@ -42,7 +42,7 @@ struct BatchInst {
 // an instance's last strip group), and blocks run in wet-first order (ws)
 // when given.
 template <int D>
-__global__ __launch_bounds__(64 * FUSED_WAVES) void batch_step(const BatchInst* __restrict__ inst,
+__global__ __launch_bounds__(64 * FUSED_WAVES) __attribute__((amdgpu_waves_per_eu(D == 1 ? 3 : 2))) void batch_step(const BatchInst* __restrict__ inst,
                                                                  const int4* __restrict__ items, int n_items,
                                                                  int n_blocks, int per_xcd, int rpw,
                                                                  const double* __restrict__ z,

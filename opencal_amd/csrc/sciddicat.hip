@@ -263,6 +263,38 @@ int stride_grid(size_t n, int block)
 	return (int)(b > 16384 ? 16384 : (b < 1 ? 1 : b));
 }
 
+// Device check of div_count: the refined reciprocal of every count 1..5 is
+// within 2^-40 of 1/b (the bound its exactness argument needs), and for n
+// seeded doubles (SplitMix64 bit patterns, every other one replaced by a
+// value of an altitude's magnitude) and every count 0..5 the bits of
+// div_count equal the IEEE quotient's (NaNs compare equal to NaNs); failures
+// are counted into *bad.
+__global__ __launch_bounds__(256) void check_div_count(long long n, unsigned long long seed,
+                                                       unsigned long long* bad)
+{
+	unsigned long long local = 0;
+	if (blockIdx.x == 0 && threadIdx.x < 5) {
+		const double b = (double)(threadIdx.x + 1);
+		local += fabs(__builtin_fma(b, count_reciprocal(b), -1.0)) <= 0x1p-40 ? 0u : 1u;
+	}
+	for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+	     k += (long long)gridDim.x * blockDim.x) {
+		unsigned long long x = seed + (unsigned long long)k * 0x9e3779b97f4a7c15ULL;
+		x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+		x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+		x ^= x >> 31;
+		double a = __longlong_as_double((long long)x);
+		if (k & 1) a = (double)(x >> 11) * 0x1p-53 * 4000.0 - 100.0;
+		for (int c = 0; c <= 5; ++c) {
+			const double e = c ? a / (double)c : a;
+			const double g = div_count(a, c);
+			const bool same = __double_as_longlong(e) == __double_as_longlong(g) || (e != e && g != g);
+			local += same ? 0u : 1u;
+		}
+	}
+	if (local) atomicAdd(bad, local);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ program
@@ -720,4 +752,23 @@ extern "C" int calhipBindSciddicaTSteering2D(struct CALModel2D* ca2D, void (*ste
 	}
 	m->prog_steering = steering;
 	return 0;
+}
+
+extern "C" long long calhipSciddicaTCheckDivision(long long n, unsigned long long seed)
+{
+	if (n < 0) {
+		set_error(25, "calhipSciddicaTCheckDivision: negative sample count");
+		return -1;
+	}
+	unsigned long long* d_bad = nullptr;
+	unsigned long long bad = 0;
+	if (CALHIP_CHECK(hipMalloc(&d_bad, sizeof bad))) return -1;
+	int rc = CALHIP_CHECK(hipMemset(d_bad, 0, sizeof bad));
+	if (!rc) {
+		hipLaunchKernelGGL(check_div_count, dim3(2048), dim3(256), 0, 0, n, seed, d_bad);
+		rc = CALHIP_CHECK(hipGetLastError());
+	}
+	if (!rc) rc = CALHIP_CHECK(hipMemcpy(&bad, d_bad, sizeof bad, hipMemcpyDeviceToHost));
+	hipFree(d_bad);
+	return rc ? -1 : (long long)bad;
 }

@@ -56,20 +56,25 @@ __device__ __forceinline__ void quad_flows(double ha, double za, double ua1, dou
 	unsigned ea = 0u, eb = 0u, ex = 0u, ey = 0u;
 	double avg_a = 0.0, avg_b = 0.0, avg_x = 0.0, avg_y = 0.0;
 	bool ga = wa, gb = wb, gx = wx, gy = wy;
-	while (ga || gb || gx || gy) {
-		bool na, nb;
-		avg_pass(u_a, ma, ea, avg_a, na);
-		avg_pass(u_b, mb, eb, avg_b, nb);
-		ga = ga && na;
-		gb = gb && nb;
-		if (__any((int)(gx || gy))) {
-			bool nx, ny;
-			avg_pass(u_x, mx, ex, avg_x, nx);
-			avg_pass(u_y, my, ey, avg_y, ny);
-			gx = gx && nx;
-			gy = gy && ny;
+	const bool fin = (!wa || levels_finite(u_a)) && (!wb || levels_finite(u_b)) && (!wx || levels_finite(u_x)) &&
+	                 (!wy || levels_finite(u_y));
+	with_finiteness(fin, [&](auto fin_t) {
+		constexpr bool F = decltype(fin_t)::value;
+		while (ga || gb || gx || gy) {
+			bool na, nb;
+			avg_pass<F>(u_a, ma, ea, avg_a, na);
+			avg_pass<F>(u_b, mb, eb, avg_b, nb);
+			ga = ga && na;
+			gb = gb && nb;
+			if (__any((int)(gx || gy))) {
+				bool nx, ny;
+				avg_pass<F>(u_x, mx, ex, avg_x, nx);
+				avg_pass<F>(u_y, my, ey, avg_y, ny);
+				gx = gx && nx;
+				gy = gy && ny;
+			}
 		}
-	}
+	});
 	if (wa) {
 #pragma unroll
 		for (int n = 1; n < 5; ++n) o.a[n - 1] = ((ea >> n) & 1u) ? 0.0 : (avg_a - u_a[n]) * r;

@@ -4,14 +4,89 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace calhip {
 namespace sct {
 
-// avg / cnt exactly as the reference divides (sciddicaT.c:72-73): one IEEE
-// division for every count, which keeps the wave's lanes on one path.
-__device__ __forceinline__ double div_count(double a, int cnt) { return a / (double)cnt; }
+// a / cnt for cnt = 0..5 (0 leaves a unchanged), bit-equal to the IEEE
+// division of the reference (sciddicaT.c:72-73) for every double a, in a
+// chain of one multiply and two FMAs instead of the division's scale /
+// reciprocal / Newton / fix-up sequence.  y is the hardware reciprocal of
+// b = cnt refined by one Newton step, within 2^-40 of 1/b (checked on the
+// device for b = 1..5 by calhipSciddicaTCheckDivision).  Then q = RN(a*y) is
+// within 2^13 ulp of Q = a/cnt, so rem = a - b*q is a multiple of q's ulp of
+// at most 18 bits and the FMA gets it exactly, and q + rem*y differs from Q
+// by less than 2^-27 ulp.  Q sits at a multiple of 1/cnt of the ulp grid, at
+// least a tenth of an ulp from every rounding midpoint (or on the grid), so
+// rounding q + rem*y once gives RN(Q) (Markstein's correction step; the same
+// holds on the subnormal grid).  rem == 0: q is exact (and keeps the sign of
+// a zero a); rem NaN: a is infinite or NaN and q is its quotient.
+__device__ __forceinline__ double count_reciprocal(double b)
+{
+	const double y0 = __builtin_amdgcn_rcp(b);
+	return __builtin_fma(y0, __builtin_fma(-b, y0, 1.0), y0);
+}
+
+__device__ __forceinline__ double div_count(double a, int cnt)
+{
+	const double b = (double)max(cnt, 1);
+	const double y = count_reciprocal(b);
+	const double q = a * y;
+	const double rem = __builtin_fma(-q, b, a);
+	const double qc = __builtin_fma(rem, y, q);
+	// class mask 0x63: signalling NaN, quiet NaN, -0, +0
+	return __builtin_amdgcn_class(rem, 0x63) ? q : qc;
+}
+
+// One pass of the reference's do-while (sciddicaT.c:62-79) on one cell:
+// avg = (m + the levels u[n] still in the set, in index order) / their count,
+// then every level of the set with avg <= u[n] leaves it (bit n of elim, the
+// reference's eliminated_cells); again = some level left.
+//
+// The sum starts at m = h - eps > 0, so it is never -0 and adding +-0.0 for
+// an eliminated level leaves it unchanged (for a dry cell nothing is used).
+// FIN (every level finite): each term is one FMA, u[n] * (0.0 or 1.0) + s,
+// which rounds exactly as s + u[n] or returns s.  Otherwise an eliminated
+// level enters as a selected +0.0.  Levels that were already eliminated may
+// be marked again: only the new ones make `again`.
+template <bool FIN>
+__device__ __forceinline__ void avg_pass(const double u[5], double m, unsigned& elim, double& avg, bool& again)
+{
+	double s = m;
+#pragma unroll
+	for (int n = 0; n < 5; ++n) {
+		if (FIN)
+			s = __builtin_fma(u[n], (double)__builtin_amdgcn_ubfe(~elim, n, 1), s);
+		else
+			s += ((elim >> n) & 1u) ? 0.0 : u[n];
+	}
+	avg = div_count(s, 5 - __builtin_popcount(elim));
+	unsigned out = elim;
+#pragma unroll
+	for (int n = 0; n < 5; ++n) out |= avg <= u[n] ? 1u << n : 0u;
+	again = out != elim;
+	elim = out;
+}
+
+__device__ __forceinline__ bool levels_finite(const double u[5])
+{
+	return __builtin_isfinite(u[0]) && __builtin_isfinite(u[1]) && __builtin_isfinite(u[2]) &&
+	       __builtin_isfinite(u[3]) && __builtin_isfinite(u[4]);
+}
+
+// Runs body(std::true_type) when every lane's levels (of the cells that take
+// part) are finite, body(std::false_type) otherwise; uniform per wave.
+template <class Body>
+__device__ __forceinline__ void with_finiteness(bool lane_finite, Body body)
+{
+	if (__all((int)lane_finite))
+		body(std::true_type{});
+	else
+		body(std::false_type{});
+}
 
 // Outflows of a wet cell (h0 > eps) with altitude z0 and neighbour levels
 // u1..u4 (N, W, E, S) = z + h of each neighbour; keep = mask of the
@@ -25,33 +100,14 @@ __device__ __forceinline__ FlowOut flows_wet_body(double h0, double z0, double u
                                                   double r, double eps)
 {
 	const double m = h0 - eps;
-	double u[5];
-	u[0] = z0 + eps;
-	u[1] = u1;
-	u[2] = u2;
-	u[3] = u3;
-	u[4] = u4;
+	const double u[5] = {z0 + eps, u1, u2, u3, u4};
 	unsigned elim = 0u;
-	double avg;
-	bool again;
-	do {
-		again = false;
-		avg = m;
-		int cnt = 0;
-#pragma unroll
-		for (int n = 0; n < 5; ++n)
-			if (!((elim >> n) & 1u)) {
-				avg += u[n];
-				cnt++;
-			}
-		if (cnt != 0) avg = div_count(avg, cnt);
-#pragma unroll
-		for (int n = 0; n < 5; ++n)
-			if (avg <= u[n] && !((elim >> n) & 1u)) {
-				elim |= 1u << n;
-				again = true;
-			}
-	} while (again);
+	double avg = 0.0;
+	with_finiteness(levels_finite(u), [&](auto fin) {
+		bool again;
+		do avg_pass<decltype(fin)::value>(u, m, elim, avg, again);
+		while (again);
+	});
 	FlowOut o;
 	o.f0 = (elim & 2u) ? 0.0 : (avg - u[1]) * r;
 	o.f1 = (elim & 4u) ? 0.0 : (avg - u[2]) * r;
@@ -67,26 +123,6 @@ __device__ __forceinline__ FlowOut flows_wet_body(double h0, double z0, double u
 struct PairFlows {
 	double a[4], b[4];
 };
-
-__device__ __forceinline__ void avg_pass(const double u[5], double m, unsigned& elim, double& avg, bool& again)
-{
-	again = false;
-	avg = m;
-	int cnt = 0;
-#pragma unroll
-	for (int n = 0; n < 5; ++n)
-		if (!((elim >> n) & 1u)) {
-			avg += u[n];
-			cnt++;
-		}
-	if (cnt != 0) avg = div_count(avg, cnt);
-#pragma unroll
-	for (int n = 0; n < 5; ++n)
-		if (avg <= u[n] && !((elim >> n) & 1u)) {
-			elim |= 1u << n;
-			again = true;
-		}
-}
 
 __device__ __forceinline__ void pair_flows(double ha, double za, double ua1, double ua2, double ua3, double ua4,
                                            double hb, double zb, double ub1, double ub2, double ub3, double ub4,
@@ -105,13 +141,16 @@ __device__ __forceinline__ void pair_flows(double ha, double za, double ua1, dou
 	// A pass after convergence changes nothing (same set, same average, no
 	// new elimination), so both cells run every pass: straight-line code
 	// with two independent chains.
-	while (ga || gb) {
-		bool na, nb;
-		avg_pass(u_a, ma, ea, avg_a, na);
-		avg_pass(u_b, mb, eb, avg_b, nb);
-		ga = ga && na;
-		gb = gb && nb;
-	}
+	with_finiteness((!wa || levels_finite(u_a)) && (!wb || levels_finite(u_b)), [&](auto fin) {
+		constexpr bool F = decltype(fin)::value;
+		while (ga || gb) {
+			bool na, nb;
+			avg_pass<F>(u_a, ma, ea, avg_a, na);
+			avg_pass<F>(u_b, mb, eb, avg_b, nb);
+			ga = ga && na;
+			gb = gb && nb;
+		}
+	});
 	if (wa) {
 #pragma unroll
 		for (int n = 1; n < 5; ++n) o.a[n - 1] = ((ea >> n) & 1u) ? 0.0 : (avg_a - u_a[n]) * r;
@@ -312,18 +351,22 @@ __device__ __forceinline__ void tri_flows(double ha, double za, double ua1, doub
 	bool ga = wa, gb = wb, gc = wc;
 	// as in pair_flows (chains a and b interleave); the halo chain c is wet
 	// in at most two lanes and its pass is skipped while no lane needs it
-	while (ga || gb || gc) {
-		bool na, nb;
-		avg_pass(u_a, ma, ea, avg_a, na);
-		avg_pass(u_b, mb, eb, avg_b, nb);
-		ga = ga && na;
-		gb = gb && nb;
-		if (__any((int)gc)) {
-			bool nc;
-			avg_pass(u_c, mc, ec, avg_c, nc);
-			gc = gc && nc;
+	const bool fin = (!wa || levels_finite(u_a)) && (!wb || levels_finite(u_b)) && (!wc || levels_finite(u_c));
+	with_finiteness(fin, [&](auto fin_t) {
+		constexpr bool F = decltype(fin_t)::value;
+		while (ga || gb || gc) {
+			bool na, nb;
+			avg_pass<F>(u_a, ma, ea, avg_a, na);
+			avg_pass<F>(u_b, mb, eb, avg_b, nb);
+			ga = ga && na;
+			gb = gb && nb;
+			if (__any((int)gc)) {
+				bool nc;
+				avg_pass<F>(u_c, mc, ec, avg_c, nc);
+				gc = gc && nc;
+			}
 		}
-	}
+	});
 	if (wa) {
 #pragma unroll
 		for (int n = 1; n < 5; ++n) o.a[n - 1] = ((ea >> n) & 1u) ? 0.0 : (avg_a - u_a[n]) * r;
@@ -543,7 +586,7 @@ __device__ __forceinline__ void wet_first_record(const WetSets& ws, int L, bool 
 // ws.f_out == nullptr: plain grid order (gridDim.x = 8 * per_xcd); otherwise
 // wet-first order (gridDim.x = 8 * (per_xcd + cap)).
 template <int D>
-__global__ __launch_bounds__(64 * FUSED_WAVES) void step_fused_al(
+__global__ __launch_bounds__(64 * FUSED_WAVES) __attribute__((amdgpu_waves_per_eu(D == 1 ? 3 : 2))) void step_fused_al(
     const double* __restrict__ z, const double* __restrict__ h, double* __restrict__ hn, double* __restrict__ trash,
     int rows, int cols, int n_strips, int n_logical, int per_xcd, int rpw, double r, double eps, WetSets ws)
 {
@@ -610,7 +653,7 @@ SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, cons
 // Two steps per sweep (sciddicat_2step.hpp, flows reset between the steps):
 // the grids it supports (even cols >= 8, rows >= 8), one launch z, h -> hn,
 // and its tile shape (CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X = small-tile rows,
-// CALHIP_SCIDDICAT_2X_GROUP = small tiles per big tile, else 16 and 2).
+// CALHIP_SCIDDICAT_2X_GROUP = small tiles per big tile, else 12 and 3).
 struct Sweep2Shape {
 	int s_rows, bigk;   // small-tile rows, small tiles per big tile
 };
@@ -652,26 +695,13 @@ __device__ __forceinline__ void flow_state(double h0, double z0, double u1, doub
 	s.u[3] = u3;
 	s.u[4] = u4;
 	bool again;
-	do {
-		again = false;
-		double avg = m;
-		int cnt = 0;
-#pragma unroll
-		for (int n = 0; n < 5; ++n)
-			if (!((s.elim >> n) & 1u)) {
-				avg += s.u[n];
-				cnt++;
-			}
-		if (cnt != 0) avg = div_count(avg, cnt);
-#pragma unroll
-		for (int n = 0; n < 5; ++n)
-			if (avg <= s.u[n] && !((s.elim >> n) & 1u)) {
-				s.elim |= 1u << n;
-				again = true;
-			}
-		s.avg = avg;
-		s.cnt = cnt;
-	} while (again);
+	with_finiteness(levels_finite(s.u), [&](auto fin) {
+		do {
+			// the count and average of the last pass (which eliminated nothing)
+			s.cnt = 5 - __builtin_popcount(s.elim);
+			avg_pass<decltype(fin)::value>(s.u, m, s.elim, s.avg, again);
+		} while (again);
+	});
 }
 
 }  // namespace sct

@@ -23,6 +23,18 @@ def gpu_run(z, h, p_r, p_eps, steps, active):
     return sciddicat_run(z, h, p_r, p_eps, steps, active)
 
 
+def test_division_equals_ieee_quotient():
+    """The kernels divide the level sum by the count of remaining levels
+    (sciddicaT.c:72-73) with a multiply and two FMAs; on the device, for 2^26
+    seeded doubles (raw bit patterns: subnormals, infinities, NaNs, and values
+    of an altitude's size) and every count 0..5, the bits equal the IEEE
+    quotient's."""
+    from opencal_amd import lib
+    print("case division check 2^26 x 6", flush=True)
+    bad = lib().calhipSciddicaTCheckDivision(1 << 26, 0x5eed)
+    assert bad == 0, f"{bad} quotients differ"
+
+
 @pytest.mark.parametrize("active", [False, True])
 @pytest.mark.parametrize("rows,cols,steps,seed", [
     (64, 96, 300, 1), (97, 131, 200, 2), (128, 61, 250, 3), (200, 256, 150, 4), (3, 5, 20, 5),
