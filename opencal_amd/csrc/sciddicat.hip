@@ -168,93 +168,159 @@ __global__ void zero_at_active(FlowPtrs f, const int* __restrict__ cells, const 
 }
 
 // The active step driven by the flag grid instead of the row-major list.
-// A flag byte holds bit 0 = in the set at the start of the step, bit 1 =
-// joined the set as a receiver in this step's flows process.  The set sizes
-// are device counters, so neither launch depends on a host value and no list
-// is rebuilt between the processes; the list is rebuilt (row-major) only when
-// it is observed.  Every cell processed is one the list path would process:
-// flows visits the set (every cell while it is empty, as OpenCAL does when
-// A.cells == NULL), width and removal visit the set plus its receivers.
+// A flag byte is 1 while its cell is in the set; the receivers of a step's
+// outflows are marked in a second byte grid (recv), which the width process
+// clears again.  The grid is cut into tiles of 4 rows x 64 columns (one wave
+// per row segment) and each launch visits only listed tiles.  For step epoch
+// e (a counter the host passes, one per step):
+//   * S[p]: the tiles that hold cells of the set, each with in_set[t] == e
+//     (built from the flags when the step starts from a host-visible state,
+//     then by the width process of the step before);
+//   * X[p]: the other tiles that hold receivers, each once (tmark[t] == e).
+// The width process visits S[p] and X[p].  No launch depends on a host
+// value and no cell list is rebuilt between the processes; the row-major
+// list is rebuilt only when it is observed.  Every cell processed is one the
+// list path would process: flows visits the set (every cell while it is
+// empty, as OpenCAL does when A.cells == NULL), width and removal visit the
+// set plus its receivers (every cell while both are empty).  Counters
+// ctr[0..1] = |S[0..1]|, ctr[2..3] = |X[0..1]|.
+struct TileLists {
+	int* ctr;                // [4]
+	int* set_list[2];        // [n_tiles] each
+	int* x_list[2];          // [n_tiles] each
+	unsigned* in_set;        // [n_tiles] epoch of the set list that holds the tile
+	unsigned* tmark;         // [n_tiles] epoch of the receiver list that holds it
+	uint8_t* recv;           // [cells]
+	int ty, tx, n_tiles;     // tile rows, tile columns, tiles
+};
 
-// flows over the set; receivers get bit 1 through a 32-bit atomic OR on the
-// word that holds their flag (the flag grid is padded to whole 16-byte
-// words), and first additions are counted into *cnt_added.  Clears the two
-// counters the next step accumulates into.
-__global__ __launch_bounds__(256) void flows_flags(const double* __restrict__ z, const double* __restrict__ h,
-                                                   FlowPtrs f, uint8_t* __restrict__ flags,
-                                                   const int* __restrict__ cnt_cur, int* __restrict__ cnt_added,
-                                                   int* __restrict__ clear_a, int* __restrict__ clear_b, int rows,
-                                                   int cols, double r, double eps, int zero_dry)
+__device__ __forceinline__ int tile_id(const TileLists& tl, int i, int j) { return i / 4 + (j / 64) * tl.ty; }
+
+// The set's tiles into S[p] (ctr[p] is zero on entry).
+__global__ __launch_bounds__(256) void tiles_from_flags(const uint8_t* __restrict__ flags, TileLists tl, int p,
+                                                        unsigned e, int rows, int cols)
 {
-	if (blockIdx.x == 0 && threadIdx.x == 0) {
-		*clear_a = 0;
-		*clear_b = 0;
-	}
-	const size_t n = (size_t)rows * cols;
-	const bool all = cnt_cur[0] == 0;
-	int added = 0;
-	for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
-		if (!all && !(flags[c] & 1u)) continue;
-		const int i = (int)(c / cols), j = (int)(c % cols);
-		const double h0 = h[c];
-		if (h0 <= eps) {
-			if (zero_dry)
-				for (int q = 0; q < 4; ++q) f.f[q][c] = 0.0;
-			continue;
-		}
-		size_t nb[4];
-		nb[0] = (size_t)dev::wrap1(i - 1, rows) * cols + j;
-		nb[1] = (size_t)i * cols + dev::wrap1(j - 1, cols);
-		nb[2] = (size_t)i * cols + dev::wrap1(j + 1, cols);
-		nb[3] = (size_t)dev::wrap1(i + 1, rows) * cols + j;
-		double fl[4];
-		const unsigned keep = cell_flows(h0, z[c], z[nb[0]] + h[nb[0]], z[nb[1]] + h[nb[1]], z[nb[2]] + h[nb[2]],
-		                                 z[nb[3]] + h[nb[3]], r, eps, fl);
-		for (int q = 0; q < 4; ++q) {
-			f.f[q][c] = fl[q];
-			if ((keep >> (q + 1)) & 1u) {
-				const unsigned sh = (unsigned)(nb[q] & 3u) * 8u;
-				const unsigned old = atomicOr(reinterpret_cast<unsigned*>(flags) + (nb[q] >> 2), 2u << sh);
-				if (((old >> sh) & 0xffu) == 0u) added++;
-			}
+	for (int t = blockIdx.x; t < tl.n_tiles; t += gridDim.x) {
+		const int i = (t % tl.ty) * 4 + (int)(threadIdx.x >> 6);
+		const int j = (t / tl.ty) * 64 + (int)(threadIdx.x & 63);
+		const bool in = i < rows && j < cols && flags[(size_t)i * cols + j] != 0;
+		if (__syncthreads_or((int)in) && threadIdx.x == 0) {
+			tl.set_list[p][atomicAdd(tl.ctr + p, 1)] = t;
+			tl.in_set[t] = e;
 		}
 	}
-	for (int off = 32; off > 0; off >>= 1) added += __shfl_down(added, off, 64);
-	if ((threadIdx.x & 63) == 0 && added) atomicAdd(cnt_added, added);
 }
 
-// width over the set and its receivers (every cell while both are empty),
-// in place, fused with the removal process: a visited cell stays in the set
-// (flag 1) when it was in it and not h' <= eps; its receiver bit is cleared.
-// The size of the new set is counted into *cnt_next.
-__global__ __launch_bounds__(256) void width_flags(double* __restrict__ h, FlowPtrs f, uint8_t* __restrict__ flags,
-                                                   const int* __restrict__ cnt_cur,
-                                                   const int* __restrict__ cnt_added, int* __restrict__ cnt_next,
-                                                   int rows, int cols, double eps)
+// flows over the set (every cell while it is empty).  A listed tile's cells
+// load their flag and all their levels at once.  Receivers get recv = 1 (a
+// plain byte store: every writer stores 1).  A block collects the tiles its
+// receivers lie in (its own, or the tile beyond one of its four edges) and
+// lists those not in the set in X[p].  Clears |S[1-p]|, which the width
+// process of this step fills.
+__global__ __launch_bounds__(256) void flows_tiles(const double* __restrict__ z, const double* __restrict__ h,
+                                                   FlowPtrs f, const uint8_t* __restrict__ flags, TileLists tl,
+                                                   int p, unsigned e, int rows, int cols, double r, double eps,
+                                                   int zero_dry)
 {
-	const size_t n = (size_t)rows * cols;
-	const bool all = cnt_cur[0] + cnt_added[0] == 0;
-	int kept = 0;
-	for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
-		const unsigned fl = flags[c];
-		if (!all && !(fl & 3u)) continue;
-		const int i = (int)(c / cols), j = (int)(c % cols);
-		const size_t cn = (size_t)dev::wrap1(i - 1, rows) * cols + j;
-		const size_t cw = (size_t)i * cols + dev::wrap1(j - 1, cols);
-		const size_t ce = (size_t)i * cols + dev::wrap1(j + 1, cols);
-		const size_t cs = (size_t)dev::wrap1(i + 1, rows) * cols + j;
-		double o = h[c];
-		o += f.f[3][cn] - f.f[0][c];
-		o += f.f[2][cw] - f.f[1][c];
-		o += f.f[1][ce] - f.f[2][c];
-		o += f.f[0][cs] - f.f[3][c];
-		h[c] = o;
-		const unsigned nf = ((fl & 3u) && !(o <= eps)) ? 1u : 0u;
-		if (nf != fl) flags[c] = (uint8_t)nf;
-		kept += (int)nf;
+	__shared__ unsigned sh_dirs;
+	if (blockIdx.x == 0 && threadIdx.x == 0) tl.ctr[1 - p] = 0;
+	const int n_set = tl.ctr[p];
+	const bool all = n_set == 0;
+	const int n_iter = all ? tl.n_tiles : n_set;
+	for (int k = blockIdx.x; k < n_iter; k += gridDim.x) {
+		const int t = all ? k : tl.set_list[p][k];
+		if (threadIdx.x == 0) sh_dirs = 0u;
+		__syncthreads();
+		const int i = (t % tl.ty) * 4 + (int)(threadIdx.x >> 6);
+		const int j = (t / tl.ty) * 64 + (int)(threadIdx.x & 63);
+		unsigned dirs = 0u;   // bit q: a receiver beyond edge q (N, W, E, S); bit 4: one in this tile
+		if (i < rows && j < cols) {
+			const size_t c = (size_t)i * cols + j;
+			const int iN = dev::wrap1(i - 1, rows), iS = dev::wrap1(i + 1, rows);
+			const int jW = dev::wrap1(j - 1, cols), jE = dev::wrap1(j + 1, cols);
+			size_t nb[4];
+			nb[0] = (size_t)iN * cols + j;
+			nb[1] = (size_t)i * cols + jW;
+			nb[2] = (size_t)i * cols + jE;
+			nb[3] = (size_t)iS * cols + j;
+			const bool in = all || flags[c] != 0;
+			const double h0 = h[c], z0 = z[c];
+			const double u1 = z[nb[0]] + h[nb[0]], u2 = z[nb[1]] + h[nb[1]];
+			const double u3 = z[nb[2]] + h[nb[2]], u4 = z[nb[3]] + h[nb[3]];
+			if (in) {
+				if (h0 <= eps) {
+					if (zero_dry)
+						for (int q = 0; q < 4; ++q) f.f[q][c] = 0.0;
+				} else {
+					double fl[4];
+					const unsigned keep = cell_flows(h0, z0, u1, u2, u3, u4, r, eps, fl);
+					const int ni[4] = {iN, i, i, iS}, nj[4] = {j, jW, jE, j};
+					for (int q = 0; q < 4; ++q) {
+						f.f[q][c] = fl[q];
+						if ((keep >> (q + 1)) & 1u) {
+							tl.recv[nb[q]] = 1;
+							dirs |= tile_id(tl, ni[q], nj[q]) == t ? 16u : 1u << q;
+						}
+					}
+				}
+			}
+		}
+		if (dirs) atomicOr(&sh_dirs, dirs);
+		__syncthreads();
+		if (threadIdx.x < 5 && ((sh_dirs >> threadIdx.x) & 1u)) {
+			const int tr = t % tl.ty, tc = t / tl.ty;
+			int u = t;
+			if (threadIdx.x == 0) u = (tr + tl.ty - 1) % tl.ty + tc * tl.ty;
+			if (threadIdx.x == 1) u = tr + ((tc + tl.tx - 1) % tl.tx) * tl.ty;
+			if (threadIdx.x == 2) u = tr + ((tc + 1) % tl.tx) * tl.ty;
+			if (threadIdx.x == 3) u = (tr + 1) % tl.ty + tc * tl.ty;
+			if (tl.in_set[u] != e && atomicExch(tl.tmark + u, e) != e) tl.x_list[p][atomicAdd(tl.ctr + 2 + p, 1)] = u;
+		}
 	}
-	for (int off = 32; off > 0; off >>= 1) kept += __shfl_down(kept, off, 64);
-	if ((threadIdx.x & 63) == 0 && kept) atomicAdd(cnt_next, kept);
+}
+
+// width over the set and its receivers in the tiles of S[p] and X[p] (every
+// cell while both are empty), in place, fused with the removal process: a
+// visited cell stays in the set (flag 1) when it was in it or received and
+// not h' <= eps; recv is cleared.  The tiles keeping cells of the set go to
+// S[1-p] with in_set = e + 1.  Clears |X[1-p]| for the next flows process.
+__global__ __launch_bounds__(256) void width_tiles(double* __restrict__ h, FlowPtrs f, uint8_t* __restrict__ flags,
+                                                   TileLists tl, int p, unsigned e, int rows, int cols, double eps)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0) tl.ctr[3 - p] = 0;
+	const int n_set = tl.ctr[p], n_x = tl.ctr[2 + p];
+	const bool all = n_set == 0 && n_x == 0;
+	const int n_iter = all ? tl.n_tiles : n_set + n_x;
+	for (int k = blockIdx.x; k < n_iter; k += gridDim.x) {
+		const int t = all ? k : (k < n_set ? tl.set_list[p][k] : tl.x_list[p][k - n_set]);
+		const int i = (t % tl.ty) * 4 + (int)(threadIdx.x >> 6);
+		const int j = (t / tl.ty) * 64 + (int)(threadIdx.x & 63);
+		bool kept = false;
+		if (i < rows && j < cols) {
+			const size_t c = (size_t)i * cols + j;
+			const unsigned fl = flags[c], rv = tl.recv[c];
+			if (all || fl || rv) {
+				const size_t cn = (size_t)dev::wrap1(i - 1, rows) * cols + j;
+				const size_t cw = (size_t)i * cols + dev::wrap1(j - 1, cols);
+				const size_t ce = (size_t)i * cols + dev::wrap1(j + 1, cols);
+				const size_t cs = (size_t)dev::wrap1(i + 1, rows) * cols + j;
+				double o = h[c];
+				o += f.f[3][cn] - f.f[0][c];
+				o += f.f[2][cw] - f.f[1][c];
+				o += f.f[1][ce] - f.f[2][c];
+				o += f.f[0][cs] - f.f[3][c];
+				h[c] = o;
+				const unsigned nf = ((fl || rv) && !(o <= eps)) ? 1u : 0u;
+				if (nf != fl) flags[c] = (uint8_t)nf;
+				if (rv) tl.recv[c] = 0;
+				kept = nf != 0u;
+			}
+		}
+		if (__syncthreads_or((int)kept) && threadIdx.x == 0) {
+			tl.set_list[1 - p][atomicAdd(tl.ctr + 1 - p, 1)] = t;
+			tl.in_set[t] = e + 1u;
+		}
+	}
 }
 
 int stride_grid(size_t n, int block)
@@ -355,17 +421,18 @@ struct SciddicaTProgram : Program {
 	Sweep2Shape shape2 = {0, 0}; // tiles of the two-step sweep (0: not chosen yet)
 	WetOrder wet_order2;         // its wet-first order
 
-	// flag-driven active step: set sizes [0..1] (current / next by parity)
-	// and receiver counts [2..3]; the row-major list is rebuilt on demand
-	int* d_ctr = nullptr;
+	// flag-driven active step: tile lists by parity (one allocation from
+	// tiles.ctr); the row-major list is rebuilt on demand
+	TileLists tiles = {};
+	unsigned tile_epoch = 0;     // step epoch of the tile lists (wraps after 2^32 steps)
 	int parity = 0;
-	bool ctr_valid = false;      // d_ctr[parity] holds the size of the set
+	bool ctr_valid = false;      // tiles.set_list[parity] lists the set's tiles
 	bool list_stale = false;     // m->d_cells / d_count lag behind the flags
 
 	~SciddicaTProgram() override
 	{
 		if (d_trash) hipFree(d_trash);
-		if (d_ctr) hipFree(d_ctr);
+		if (tiles.ctr) hipFree(tiles.ctr);
 	}
 
 	int sync_active(ModelX* m) override
@@ -465,36 +532,60 @@ struct SciddicaTProgram : Program {
 		}
 		const double r = *p_r, eps = *p_eps;
 		const int rows = m->pub.rows, cols = m->pub.columns;
-		const size_t n = m->cells();
-		if (!d_ctr) {
-			if (CALHIP_CHECK(hipMalloc(&d_ctr, 4 * sizeof(int)))) return -1;
-			if (CALHIP_CHECK(hipMemsetAsync(d_ctr, 0, 4 * sizeof(int), m->stream))) return -1;
+		if (!tiles.ctr) {
+			tiles.ty = (rows + 3) / 4;
+			tiles.tx = (cols + 63) / 64;
+			tiles.n_tiles = tiles.ty * tiles.tx;
+			const size_t nt = (size_t)tiles.n_tiles;
+			const size_t bytes = 4 * sizeof(int) + 4 * nt * sizeof(int) + 2 * nt * sizeof(unsigned) + m->cells();
+			char* b = nullptr;
+			if (CALHIP_CHECK(hipMalloc(&b, bytes))) return -1;
+			if (CALHIP_CHECK(hipMemsetAsync(b, 0, bytes, m->stream))) {
+				hipFree(b);
+				return -1;
+			}
+			tiles.ctr = reinterpret_cast<int*>(b);
+			int* lists = tiles.ctr + 4;
+			tiles.set_list[0] = lists;
+			tiles.set_list[1] = lists + nt;
+			tiles.x_list[0] = lists + 2 * nt;
+			tiles.x_list[1] = lists + 3 * nt;
+			tiles.in_set = reinterpret_cast<unsigned*>(lists + 4 * nt);
+			tiles.tmark = tiles.in_set + nt;
+			tiles.recv = reinterpret_cast<uint8_t*>(tiles.tmark + nt);
 			ctr_valid = false;
 		}
-		if (!ctr_valid) {
-			// the list is current: its length is the size of the set
-			if (sync_active(m)) return -1;
-			if (CALHIP_CHECK(hipMemcpyAsync(d_ctr + parity, m->d_count, sizeof(int), hipMemcpyDeviceToDevice,
-			                                m->stream)) ||
-			    CALHIP_CHECK(hipMemsetAsync(d_ctr + 2 + parity, 0, sizeof(int), m->stream)))
+		const int grid = std::min(tiles.n_tiles, 1024);
+		hipEvent_t ev;
+		timer_begin(m, &ev);
+		if (tile_epoch >= 0xfffffff0u) {
+			// the epochs wrap: start the tags again from the arrays' zero state
+			if (CALHIP_CHECK(hipMemsetAsync(tiles.in_set, 0, 2 * (size_t)tiles.n_tiles * sizeof(unsigned),
+			                                m->stream)))
 				return -1;
+			tile_epoch = 0;
+			ctr_valid = false;
+		}
+		// a rebuilt list needs an epoch no tag holds yet: the last width
+		// process tagged its tiles with (last epoch + 1)
+		if (!ctr_valid) tile_epoch++;
+		const unsigned e = ++tile_epoch;
+		if (!ctr_valid) {
+			// the flags are current: list the tiles that hold cells of the set
+			if (CALHIP_CHECK(hipMemsetAsync(tiles.ctr, 0, 4 * sizeof(int), m->stream))) return -1;
+			hipLaunchKernelGGL(tiles_from_flags, dim3(grid), dim3(256), 0, m->stream, (const uint8_t*)m->d_flags,
+			                   tiles, parity, e, rows, cols);
+			m->stats.launches++;
 			ctr_valid = true;
 		}
 		int zero_dry = 1;
 		for (int q = 0; q < 4; ++q)
 			if (f[q]->tstate != Transient::ZeroAtActive && f[q]->tstate != Transient::Zero) zero_dry = 0;
-		int* cur = d_ctr + parity;
-		int* nxt = d_ctr + (1 - parity);
-		int* added = d_ctr + 2 + parity;
-		int* added_next = d_ctr + 2 + (1 - parity);
-		const int grid = (int)std::min<size_t>((n + 255) / 256, 65535);
-		hipEvent_t ev;
-		timer_begin(m, &ev);
-		hipLaunchKernelGGL(flows_flags, dim3(grid), dim3(256), 0, m->stream, (const double*)z->d_cur,
-		                   (const double*)h->d_cur, cur_flows(), m->d_flags, (const int*)cur, added, nxt,
-		                   added_next, rows, cols, r, eps, zero_dry);
-		hipLaunchKernelGGL(width_flags, dim3(grid), dim3(256), 0, m->stream, (double*)h->d_cur, cur_flows(),
-		                   m->d_flags, (const int*)cur, (const int*)added, nxt, rows, cols, eps);
+		hipLaunchKernelGGL(flows_tiles, dim3(grid), dim3(256), 0, m->stream, (const double*)z->d_cur,
+		                   (const double*)h->d_cur, cur_flows(), (const uint8_t*)m->d_flags, tiles, parity, e,
+		                   rows, cols, r, eps, zero_dry);
+		hipLaunchKernelGGL(width_tiles, dim3(grid), dim3(256), 0, m->stream, (double*)h->d_cur, cur_flows(),
+		                   m->d_flags, tiles, parity, e, rows, cols, eps);
 		timer_end(m, ev);
 		m->stats.launches += 2;
 		if (CALHIP_CHECK(hipGetLastError())) return -1;
