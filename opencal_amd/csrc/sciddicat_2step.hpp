@@ -152,6 +152,13 @@ __device__ __forceinline__ bool sweep2_strip_al(const double* __restrict__ z, co
 	double sa0 = 0.0, sa1 = 0.0, sa2 = 0.0, sa3 = 0.0, sb0 = 0.0, sb1 = 0.0, sb2 = 0.0, sb3 = 0.0;
 	double sc1 = 0.0, sc2 = 0.0, spa = 0.0, spb = 0.0;
 	bool seen = false;
+	// Dry rows.  A cell with h <= eps has no outflow (the process returns
+	// before writing them), so a row whose cells in this wave are all dry has
+	// zero flows, and a row whose own flows and those of the rows on either
+	// side are zero keeps its level: h' = h + (0 - 0) + ... = h + 0.0, the
+	// same bits the full update gives.  Bit k of hist_a / hist_b: the flows
+	// of the stage's position k rows back may be non-zero (wave-uniform).
+	unsigned hist_a = 0u, hist_b = 0u;
 
 	for (int base = 0; base < na; base += U) {
 #pragma unroll
@@ -169,30 +176,42 @@ __device__ __forceinline__ bool sweep2_strip_al(const double* __restrict__ z, co
 			{
 				const int sb = d % U, sc = (d + 1) % U, sa = (d + 2) % U;
 				const double2 zb = bz[sb], hb = bh[sb], zc = bz[sc], hc = bh[sc], za = bz[sa], ha = bh[sa];
-				const double2 zN = UP ? za : zb, hN = UP ? ha : hb, zS = UP ? zb : za, hS = UP ? hb : ha;
 				const double2 xzb = cz[sb], xhb = ch[sb], xzc = cz[sc], xhc = ch[sc], xza = cz[sa], xha = ch[sa];
-				const double2 xzN = UP ? xza : xzb, xhN = UP ? xha : xhb, xzS = UP ? xzb : xza,
-				              xhS = UP ? xhb : xha;
-				const double ua = zc.x + hc.x, ub = zc.y + hc.y;
-				const double ux = xzc.x + xhc.x, uy = xzc.y + xhc.y;
-				// horizontal neighbour levels
-				const double ua_first = lane_value(ua, 0);    // column j0
-				const double ub_last = lane_value(ub, kR);    // the strip's last column
-				const double uy_l1 = lane_value(uy, 1);       // column j0-1
-				const double ux_l2 = lane_value(ux, 2);       // column jR
-				double uw = dev::dpp_from_left(ub);
-				if (lane0) uw = uy_l1;
-				double ue = dev::dpp_from_right(ua);
-				if (lane == kR) ue = ux_l2;
-				double uxw = dev::dpp_from_left(uy);         // x's west: y of lane-1
-				if (lane == 2) uxw = ub_last;
-				double uye = dev::dpp_from_right(ux);        // y's east: x of lane+1
-				if (lane == 1) uye = ua_first;
+				const bool wet_a = __any((int)(!(hc.x <= eps) || !(hc.y <= eps) || (x_on && !(xhc.x <= eps)) ||
+				                               (y_on && !(xhc.y <= eps)))) != 0;
+				hist_a = ((hist_a << 1) | (wet_a ? 1u : 0u)) & 7u;
 				QuadFlows f;
-				quad_flows(hc.x, zc.x, zN.x + hN.x, uw, ub, zS.x + hS.x, hc.y, zc.y, zN.y + hN.y, ua, ue,
-				           zS.y + hS.y, x_on, xhc.x, xzc.x, xzN.x + xhN.x, uxw, uy, xzS.x + xhS.x, y_on, xhc.y,
-				           xzc.y, xzN.y + xhN.y, ux, uye, xzS.y + xhS.y, r, eps, f, seen);
+				if (wet_a) {
+					const double2 zN = UP ? za : zb, hN = UP ? ha : hb, zS = UP ? zb : za, hS = UP ? hb : ha;
+					const double2 xzN = UP ? xza : xzb, xhN = UP ? xha : xhb, xzS = UP ? xzb : xza,
+					              xhS = UP ? xhb : xha;
+					const double ua = zc.x + hc.x, ub = zc.y + hc.y;
+					const double ux = xzc.x + xhc.x, uy = xzc.y + xhc.y;
+					// horizontal neighbour levels
+					const double ua_first = lane_value(ua, 0);    // column j0
+					const double ub_last = lane_value(ub, kR);    // the strip's last column
+					const double uy_l1 = lane_value(uy, 1);       // column j0-1
+					const double ux_l2 = lane_value(ux, 2);       // column jR
+					double uw = dev::dpp_from_left(ub);
+					if (lane0) uw = uy_l1;
+					double ue = dev::dpp_from_right(ua);
+					if (lane == kR) ue = ux_l2;
+					double uxw = dev::dpp_from_left(uy);         // x's west: y of lane-1
+					if (lane == 2) uxw = ub_last;
+					double uye = dev::dpp_from_right(ux);        // y's east: x of lane+1
+					if (lane == 1) uye = ua_first;
+					quad_flows(hc.x, zc.x, zN.x + hN.x, uw, ub, zS.x + hS.x, hc.y, zc.y, zN.y + hN.y, ua, ue,
+					           zS.y + hS.y, x_on, xhc.x, xzc.x, xzN.x + xhN.x, uxw, uy, xzS.x + xhS.x, y_on,
+					           xhc.y, xzc.y, xzN.y + xhN.y, ux, uye, xzS.y + xhS.y, r, eps, f, seen);
+				} else {
+#pragma unroll
+					for (int k = 0; k < 4; ++k) f.a[k] = f.b[k] = f.x[k] = f.y[k] = 0.0;
+				}
 				// h1 of position q (the row behind)
+				if (hist_a == 0u) {
+					g1[d % 3] = make_double2(hb.x + 0.0, hb.y + 0.0);
+					g1h[d % 3] = make_double2(xhb.x + 0.0, xhb.y + 0.0);
+				} else {
 				const double nA = UP ? f.a[3] : ppa, sA = UP ? ppa : f.a[0];
 				const double nB = UP ? f.b[3] : ppb, sB = UP ? ppb : f.b[0];
 				const double nX = UP ? f.x[3] : ppx, sX = UP ? ppx : f.x[0];
@@ -228,6 +247,7 @@ __device__ __forceinline__ bool sweep2_strip_al(const double* __restrict__ z, co
 				oy += sY - py3;
 				g1[d % 3] = make_double2(oa, ob);   // slot of position q (U is a multiple of 3)
 				g1h[d % 3] = make_double2(ox, oy);
+				}
 				ppa = UP ? pa0 : pa3;
 				ppb = UP ? pb0 : pb3;
 				ppx = UP ? px0 : px3;
@@ -256,6 +276,14 @@ __device__ __forceinline__ bool sweep2_strip_al(const double* __restrict__ z, co
 				const int gb = (d + 1) % 3, gc = (d + 2) % 3, ga = d % 3;           // h1 slots of q-2, q-1, q
 				const double2 zb = bz[sb], zc = bz[sc], za = bz[sa];
 				const double2 hb = g1[gb], hc = g1[gc], ha = g1[ga];
+				// the stage-B halo cells are j0-1 (y of lane 1's halo pair) and jR
+				// (x of lane 2's), moved to lanes 0 and 1 below
+				const bool wet_b = __any((int)(!(hc.x <= eps) || !(hc.y <= eps) ||
+				                               (lane == 1 && !(g1h[gc].y <= eps)) ||
+				                               (lane == 2 && !(g1h[gc].x <= eps)))) != 0;
+				hist_b = ((hist_b << 1) | (wet_b ? 1u : 0u)) & 7u;
+				TriFlows tf;
+				if (wet_b) {
 				const double2 zN = UP ? za : zb, hN = UP ? ha : hb, zS = UP ? zb : za, hS = UP ? hb : ha;
 				// halo pairs (j0-2, j0-1) to lane 0 and (jR, jR+1) to lane 1
 				const double2 czb = make_double2(dev::dpp_from_right(cz[sb].x), dev::dpp_from_right(cz[sb].y));
@@ -278,9 +306,17 @@ __device__ __forceinline__ bool sweep2_strip_al(const double* __restrict__ z, co
 				const double uS_c = lane0 ? czS.y + chS.y : czS.x + chS.x;
 				const double uW_c = lane0 ? uhx : ub_last;
 				const double uE_c = lane0 ? ua : uhy;
-				TriFlows tf;
 				tri_flows(hc.x, zc.x, zN.x + hN.x, uw, ub, zS.x + hS.x, hc.y, zc.y, zN.y + hN.y, ua, ue,
 				          zS.y + hS.y, c_on, hc_c, zc_c, uN_c, uW_c, uE_c, uS_c, r, eps, tf, seen);
+				} else {
+#pragma unroll
+					for (int k = 0; k < 4; ++k) tf.a[k] = tf.b[k] = tf.c[k] = 0.0;
+				}
+				double oa, ob;
+				if (hist_b == 0u) {
+					oa = hb.x + 0.0;
+					ob = hb.y + 0.0;
+				} else {
 				const double nA = UP ? tf.a[3] : spa, sA = UP ? spa : tf.a[0];
 				const double nB = UP ? tf.b[3] : spb, sB = UP ? spb : tf.b[0];
 				double f2w = dev::dpp_from_left(sb2);
@@ -288,7 +324,8 @@ __device__ __forceinline__ bool sweep2_strip_al(const double* __restrict__ z, co
 				const double sc1_right = lane_value(sc1, 1);
 				double f1e = dev::dpp_from_right(sa1);
 				if (lane == kR) f1e = sc1_right;
-				double oa = hb.x, ob = hb.y;
+				oa = hb.x;
+				ob = hb.y;
 				oa += nA - sa0;
 				oa += f2w - sa1;
 				oa += sb1 - sa2;
@@ -297,6 +334,7 @@ __device__ __forceinline__ bool sweep2_strip_al(const double* __restrict__ z, co
 				ob += sa2 - sb1;
 				ob += f1e - sb2;
 				ob += sB - sb3;
+				}
 				const bool row_ok = t >= 2;
 				// output row: stage-B position t = stream position q-2
 				double* dst = (row_ok && own) ? hn + (size_t)row_of(q - 2) * (size_t)cols + j : trash + 2 * lane;
