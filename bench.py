@@ -374,7 +374,8 @@ def main():
                      "bytes_per_cell_update": BYTES_PER_CELL_SCIDDICAT,
                      "note": "achieved = 24 B per cell update x cell updates / kernel time (the contract's "
                              "algorithmic bytes); a two-step launch moves about 24 B per cell for two updates, "
-                             "so its HBM traffic (traffic) is about half of those algorithmic bytes"},
+                             "so its HBM traffic (traffic, per launch) is about half of those algorithmic bytes; "
+                             "traffic_gbs / traffic_frac = that measured traffic over the same kernel time"},
         "per_step_driver": {
             "driver": "calRunCAStep2D per step (one sweep per step)", "gcups": round(r1["gcups"], 3),
             "ms_per_step": round(r1["wall_s"] / args.steps * 1e3, 4),
@@ -384,6 +385,12 @@ def main():
                          "traffic_source": "profiles/sciddicat_sweep_4096x4096.md",
                          "kernel": "step_fused_al", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2)}},
     }
+    # the measured HBM bytes per launch over the same kernel time: the rate the
+    # memory actually sustains, beside the algorithmic rate of `achieved`
+    for rl in (out["roofline"], out["per_step_driver"]["roofline"]):
+        if rl["traffic"]:
+            rl["traffic_gbs"] = round(rl["traffic"] / (rl["kernel_us"] * 1e-6) / 1e9, 1)
+            rl["traffic_frac"] = round(rl["traffic_gbs"] / HBM_PEAK_GBS, 4)
     log("pcie-inclusive run ...")
     out["pcie_inclusive"] = pcie_inclusive_sciddicat(args.size, args.steps)
     if not args.no_cpu:

@@ -653,7 +653,7 @@ SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, cons
 // Two steps per sweep (sciddicat_2step.hpp, flows reset between the steps):
 // the grids it supports (even cols >= 8, rows >= 8), one launch z, h -> hn,
 // and its tile shape (CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X = small-tile rows,
-// CALHIP_SCIDDICAT_2X_GROUP = small tiles per big tile, else 12 and 3).
+// CALHIP_SCIDDICAT_2X_GROUP = small tiles per big tile, else 12 and 4).
 struct Sweep2Shape {
 	int s_rows, bigk;   // small-tile rows, small tiles per big tile
 };

@@ -196,12 +196,12 @@ Sweep2Shape sweep2_shape(hipStream_t st, int rows, int cols, const double* z, co
                          double* trash, double r, double eps)
 {
 	(void)st, (void)rows, (void)cols, (void)z, (void)h, (void)scratch, (void)trash, (void)r, (void)eps;
-	// Small tiles of 12 rows, big tiles of three: measured at 4096^2 (full
-	// grid, after 60 and after 500 steps of a spreading front) against
-	// 8..32-row small tiles and groups of 1..8; best at both.  Not autotuned:
-	// the best shape depends on how much of the grid is wet, which changes as
-	// the run advances.
-	Sweep2Shape s = {12, 3};
+	// Small tiles of 12 rows, big tiles of four: measured at 4096^2 (full
+	// grid, 500 steps of a spreading front) against 6..24-row small tiles and
+	// groups of 2..8 (12 x 4: 101 us per launch; 12 x 3: 107; 8 x 4: 103;
+	// 16 x 2: 112).  Not autotuned: the best shape depends on how much of the
+	// grid is wet, which changes as the run advances.
+	Sweep2Shape s = {12, 4};
 	if (const char* e = getenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X"))
 		if (atoi(e) > 0) s.s_rows = atoi(e);
 	if (const char* e = getenv("CALHIP_SCIDDICAT_2X_GROUP"))
