@@ -123,7 +123,14 @@ __device__ __forceinline__ bool set_flag(uint8_t* flags, int c)
 	return !(atomicOr(w, bit) & bit);
 }
 
-// outflowsMin + empiricalFlows (Sciara.cpp:23-105)
+// outflowsMin + empiricalFlows (Sciara.cpp:23-105), eight lanes per cell:
+// lane k-1 of an aligned group of eight handles neighbour k (its level, slope
+// angle, critical thickness and outflow), so a cell's eight atan / cos
+// evaluations run side by side instead of one after another.  The group
+// shares the eight levels through lane shuffles and every lane runs the same
+// minimisation on them (same operations, same order: the same average).
+// The grid-stride loop walks cells by group; a group's lanes always agree on
+// its cell, so a shuffle only reads lanes that take part.
 template <bool ACTIVE>
 __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ z, const double* __restrict__ h,
                                                      const double* __restrict__ T, Flows8 f, uint8_t* flags,
@@ -133,8 +140,11 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 	const size_t n = (size_t)rows * cols;
 	const int cnt0 = ACTIVE ? count[0] : 0;
 	const size_t total = sweep_total(cnt0, n);
-	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-	     idx += (size_t)gridDim.x * blockDim.x) {
+	const int lane = (int)(threadIdx.x & 63);
+	const int g0 = lane & ~7;          // first lane of this lane's group
+	const int k = (lane & 7) + 1;      // the neighbour this lane handles
+	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
+	for (size_t idx = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3); idx < total; idx += stride) {
 		const int c = cnt0 > 0 ? cells[idx] : (int)idx;
 		const double h0 = h[c];
 		if (!(h0 > 0)) continue;
@@ -143,63 +153,46 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 		const double Pr = pm_pow10(P.a + P.b * t);
 		const double hc = pm_pow10(P.c + P.d * t);
 		const double sz0 = z[c];
-		int nb[9];
-		double zz[9], hh[9], H[9], th[9];
-		th[0] = 0.0;
+		const int nbk = nb_flat(i, j, k, rows, cols);
+		const double hk = nbk >= 0 ? h[nbk] : 0.0;
+		const double zr = nbk >= 0 ? z[nbk] : 0.0;
+		const double zk = k < 5 ? zr : sz0 - (sz0 - zr) / P.rad2;
+		const double lvl0 = sz0 + h0;
+		const double lk = zk + hk;
+		const bool down = lvl0 > lk;
+		const double Hk = down ? lk : 0.0;
+		const double hcck = hc * pm_cos(pm_atan((lvl0 - lk) / P.Pc));
+		double H[9];
+		H[0] = sz0;
 #pragma unroll
-		for (int k = 0; k < 9; ++k) {
-			nb[k] = k == 0 ? c : nb_flat(i, j, k, rows, cols);
-			hh[k] = nb[k] >= 0 ? h[nb[k]] : 0.0;
-			const double zk = nb[k] >= 0 ? z[nb[k]] : 0.0;
-			zz[k] = k < 5 ? zk : sz0 - (sz0 - zk) / P.rad2;
-			H[k] = 0.0;
-		}
-		H[0] = zz[0];
-		unsigned keep = 1u;
-		const double lvl0 = zz[0] + hh[0];
-		// the slope angles and critical thicknesses of all eight neighbours
-		// are computed without branches (eight independent chains the
-		// scheduler interleaves); only those of lower neighbours are used
-		double hcc[9];
-#pragma unroll
-		for (int k = 1; k < 9; ++k) {
-			const double lk = zz[k] + hh[k];
-			const bool down = lvl0 > lk;
-			H[k] = down ? lk : 0.0;
-			th[k] = pm_atan((lvl0 - lk) / P.Pc);
-			hcc[k] = hc * pm_cos(th[k]);
-			keep |= (down ? 1u : 0u) << k;
-		}
+		for (int q = 1; q < 9; ++q) H[q] = __shfl(Hk, g0 + q - 1, 64);
+		unsigned keep = 1u | ((unsigned)((__ballot((int)down) >> g0) & 0xffull) << 1);
 		double avg;
 		bool loop;
 		do {
 			loop = false;
-			avg = hh[0];
+			avg = h0;
 			int counter = 0;
 #pragma unroll
-			for (int k = 0; k < 9; ++k)
-				if ((keep >> k) & 1u) {
-					avg += H[k];
+			for (int q = 0; q < 9; ++q)
+				if ((keep >> q) & 1u) {
+					avg += H[q];
 					counter++;
 				}
 			if (counter != 0) avg = avg / (double)counter;
 #pragma unroll
-			for (int k = 0; k < 9; ++k)
-				if (((keep >> k) & 1u) && avg <= H[k]) {
-					keep &= ~(1u << k);
+			for (int q = 0; q < 9; ++q)
+				if (((keep >> q) & 1u) && avg <= H[q]) {
+					keep &= ~(1u << q);
 					loop = true;
 				}
 		} while (loop);
-		const int base = cnt0;
-#pragma unroll
-		for (int k = 1; k < 9; ++k) {
-			if (!((keep >> k) & 1u)) continue;
-			if (!(h0 > hcc[k])) continue;
-			const double fk = Pr * (avg - H[k]);
+		if (((keep >> k) & 1u) && h0 > hcck) {
+			const double fk = Pr * (avg - Hk);
 			if (fk > 0) {
 				f.f[k - 1][c] = fk;
-				if (ACTIVE && nb[k] >= 0 && set_flag(flags, nb[k]) && append)
-					cells[base + atomicAdd(&count[1], 1)] = nb[k];
+				if (ACTIVE && nbk >= 0 && set_flag(flags, nbk) && append)
+					cells[cnt0 + atomicAdd(&count[1], 1)] = nbk;
 			}
 		}
 	}
@@ -285,6 +278,14 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_finish(double* __restrict__ h, do
 	const int lane = threadIdx.x & (dev::WAVE - 1);
 	__shared__ int wave_off[SC_BLOCK / dev::WAVE];
 	__shared__ int block_pos;
+	// the steering's border reset first: its cells are independent of the
+	// list, and where a border cell is also listed both write h = T = 0
+	if (ACTIVE && STEER)
+		for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n_mb; k += stride) {
+			const int c = mb_cells[k];
+			h[c] = 0.0;
+			T[c] = 0.0;
+		}
 	for (size_t base = (size_t)blockIdx.x * blockDim.x; base < total; base += stride) {
 		const size_t idx = base + threadIdx.x;
 		const bool valid = idx < total;
@@ -320,21 +321,17 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_finish(double* __restrict__ h, do
 		}
 	}
 	if (!ACTIVE) return;
-	if (STEER)
-		for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n_mb; k += stride) {
-			const int c = mb_cells[k];
-			h[c] = 0.0;
-			T[c] = 0.0;
-		}
+	// The last block to finish moves the survivor count to count[0].  Only
+	// the counters pass between blocks here, all through device-scope atomics
+	// (the list and the cells are read by later launches), so no fence is
+	// needed before the block's arrival is counted.
 	__shared__ int last_block;
 	__shared__ int survivors;
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		__threadfence();
 		const int done = atomicAdd(&count[3], 1);
 		last_block = done == (int)gridDim.x - 1;
 		if (last_block) {
-			__threadfence();
 			survivors = atomicAdd(&count[2], 0);
 			atomicExch(&count[0], survivors);
 			atomicExch(&count[1], 0);
@@ -554,11 +551,12 @@ struct SciaraProgram : Program {
 		}
 	}
 
-	int list_grid(ModelX* m) const
+	// blocks for `lanes` threads per listed cell
+	int list_grid(ModelX* m, int lanes = 1) const
 	{
-		if (!active) return clamp_grid(m->cells(), 1, 8192);
+		if (!active) return clamp_grid(m->cells() * (size_t)lanes, 1, 8192);
 		// room for growth between read-backs; the kernels loop over any excess
-		return clamp_grid((size_t)(est + est / 4 + 1024), 8, 8192);
+		return clamp_grid((size_t)(est + est / 4 + 1024) * (size_t)lanes, 8, 8192);
 	}
 
 	int transition(ModelX* m, bool steer)
@@ -579,12 +577,13 @@ struct SciaraProgram : Program {
 			unordered = false;
 		}
 		const int grid = list_grid(m);
+		const int grid8 = list_grid(m, 8);   // eight lanes per cell
 		if (active)
-			hipLaunchKernelGGL(sc_flows<true>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (const double*)z->d_cur,
+			hipLaunchKernelGGL(sc_flows<true>, dim3(grid8), dim3(SC_BLOCK), 0, m->stream, (const double*)z->d_cur,
 			                   (const double*)h->d_cur, (const double*)T->d_cur, F, m->d_flags, m->d_cells,
 			                   m->d_count, rows, cols, P, 1);
 		else
-			hipLaunchKernelGGL(sc_flows<false>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (const double*)z->d_cur,
+			hipLaunchKernelGGL(sc_flows<false>, dim3(grid8), dim3(SC_BLOCK), 0, m->stream, (const double*)z->d_cur,
 			                   (const double*)h->d_cur, (const double*)T->d_cur, F, nullptr, nullptr, nullptr,
 			                   rows, cols, P, 0);
 		double* Tn = (double*)T->d_nxt;
