@@ -24,6 +24,108 @@
 namespace calhip {
 namespace sct {
 
+// Tiles of the two-step sweep.  A small tile is `s_rows` rows of a group of
+// FUSED_WAVES strips (one wave per strip); a big tile is `bigk` small tiles
+// stacked vertically.  A wave that meets wet cells pays the minimisation's
+// latency on every row of its chunk, twice per step pair, while a dry chunk
+// is cheaper per row the longer it is (its four halo rows above and below
+// are re-read and recomputed).  So the launch runs the small tiles that met
+// wet cells in the previous launch first, one block each (wet-first order,
+// as in step_fused_al), and every big tile as one chunk per wave of its
+// runs of small tiles that are not in that list.  Every small tile is
+// computed exactly once per launch, so the order and the grouping never
+// change a result.  Each small tile records whether it met a wet cell (a
+// run that met one marks all its small tiles).
+struct Sweep2Geom {
+	int n_strips, n_sg, s_rows, bigk, n_small_chunks, n_small, n_big, per_xcd;
+};
+
+// Records whether small tile S met a wet cell; a wet tile is listed for the
+// next launch's wet blocks (a full list leaves it in its big tile).
+__device__ __forceinline__ void wet2_record(const WetSets& ws, int grp, int S, bool wet)
+{
+	if (wet) {
+		const unsigned jx = atomicAdd(ws.n_out + grp, 1u);
+		if ((int)jx < ws.cap)
+			ws.l_out[grp * ws.cap + jx] = S;
+		else
+			wet = false;
+	}
+	ws.f_out[S] = (unsigned char)(wet ? 1 : 0);
+}
+
+template <bool UP>
+__device__ __forceinline__ bool sweep2_rows(const double* z, const double* h, double* hn, double* trash, int rows,
+                                            int cols, int strip, int r0, int r1, double r, double eps)
+{
+	return sweep2_strip_al<UP>(z, h, hn, trash, rows, cols, cols, strip, r0, r1, r, eps);
+}
+
+// ws.f_out == nullptr: big tiles only (gridDim.x = 8 * per_xcd); otherwise
+// gridDim.x = 8 * (per_xcd + ws.cap), the first ws.cap blocks of each XCD
+// group taking the listed small tiles.
+__global__ __launch_bounds__(64 * FUSED_WAVES) __attribute__((amdgpu_waves_per_eu(2, 2))) void step_fused_2x(
+    const double* __restrict__ z, const double* __restrict__ h, double* __restrict__ hn, double* __restrict__ trash,
+    int rows, int cols, Sweep2Geom g, double r, double eps, WetSets ws)
+{
+	const int grp = blockIdx.x & 7, l = blockIdx.x >> 3;
+	const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	__shared__ unsigned blk_mask;
+	if (threadIdx.x == 0) blk_mask = 0u;
+	if (ws.f_out && blockIdx.x == 0 && threadIdx.x < 8) ws.n_zero[threadIdx.x] = 0u;
+	__syncthreads();
+	const int nw = ws.f_out ? min((int)ws.n_in[grp], ws.cap) : 0;
+	if (l < nw) {
+		const int S = ws.l_in[grp * ws.cap + l];
+		const int cs = S / g.n_sg, sg = S % g.n_sg;
+		const int strip = sg * FUSED_WAVES + w;
+		bool wet = false;
+		if (strip < g.n_strips) {
+			const int r0 = cs * g.s_rows, r1 = min(r0 + g.s_rows, rows);
+			wet = (cs & 1) ? sweep2_rows<true>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps)
+			               : sweep2_rows<false>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
+		}
+		if (wet && lane == 0) atomicOr(&blk_mask, 1u);
+		__syncthreads();
+		if (threadIdx.x == 0) wet2_record(ws, grp, S, blk_mask != 0u);
+		return;
+	}
+	const int k = l - nw;
+	if (k >= g.per_xcd) return;
+	const int B = grp * g.per_xcd + k;
+	if (B >= g.n_big) return;
+	const int cb = B / g.n_sg, sg = B % g.n_sg;
+	const int strip = sg * FUSED_WAVES + w;
+	const int cs0 = cb * g.bigk, cs1 = min(cs0 + g.bigk, g.n_small_chunks);
+	unsigned listed = 0u;   // small tiles of this big tile that the wet blocks take
+	if (ws.f_out)
+		for (int c = cs0; c < cs1; ++c)
+			if (ws.f_in[c * g.n_sg + sg]) listed |= 1u << (c - cs0);
+	unsigned my_wet = 0u;
+	if (strip < g.n_strips) {
+		int c = cs0;
+		while (c < cs1) {
+			if ((listed >> (c - cs0)) & 1u) {
+				++c;
+				continue;
+			}
+			int e = c + 1;
+			while (e < cs1 && !((listed >> (e - cs0)) & 1u)) ++e;
+			const int r0 = c * g.s_rows, r1 = min(e * g.s_rows, rows);
+			const bool wet = (c & 1) ? sweep2_rows<true>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps)
+			                         : sweep2_rows<false>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
+			if (wet) my_wet |= ((1u << (e - c)) - 1u) << (c - cs0);
+			c = e;
+		}
+	}
+	if (my_wet && lane == 0) atomicOr(&blk_mask, my_wet);
+	__syncthreads();
+	if (threadIdx.x == 0 && ws.f_out)
+		for (int c = cs0; c < cs1; ++c)
+			if (!((listed >> (c - cs0)) & 1u)) wet2_record(ws, grp, c * g.n_sg + sg, (blk_mask >> (c - cs0)) & 1u);
+}
+
+
 SweepGeom sweep_geom(int rows, int cols, int rpw)
 {
 	SweepGeom g;

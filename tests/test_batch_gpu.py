@@ -10,18 +10,22 @@ from opencal_amd import synth
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_vs_oracle():
+@pytest.mark.parametrize("steps", [150, 151])
+def test_batch_vs_oracle(steps):
+    """Uneven sizes, including grids shorter than a chunk and narrower than
+    a strip, over an even and an odd number of steps."""
     from opencal_amd.models import sciddicat_batch
     specs = synth.batch_instances(12, seed=77, lo=24, hi=180)
+    specs += [(6, 40, 0.5, 0.002, 901), (20, 6, 0.45, 0.001, 902), (9, 8, 0.6, 0.004, 903), (8, 130, 0.5, 0.001, 904)]
     insts = []
     for rows, cols, pr, pe, seed in specs:
         z, h, _ = synth.sciddicat_inputs(rows, cols, seed, patch=min(12, rows // 4 or 1))
         insts.append((z, h, pr, pe))
-    print("case batch sizes", [(r, c) for r, c, *_ in specs], flush=True)
-    outs, ms = sciddicat_batch(insts, 150)
+    print("case batch steps", steps, "sizes", [(r, c) for r, c, *_ in specs], flush=True)
+    outs, ms = sciddicat_batch(insts, steps)
     assert ms > 0
     for (z, h, pr, pe), got in zip(insts, outs):
-        _, want = native.oracle_sciddicat(z, h, pr, pe, 150, False)
+        _, want = native.oracle_sciddicat(z, h, pr, pe, steps, False)
         assert np.array_equal(got, want)
 
 
