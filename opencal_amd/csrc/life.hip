@@ -18,6 +18,9 @@
 //               from every side of the region) is exact.  Used by calRun2D
 //               when no host callback runs between steps; the first step
 //               runs per cell so the board holds only 0/1 values.
+#include <stdio.h>
+#include <stdlib.h>
+
 #include "engine.hpp"
 #include "device_common.hpp"
 
@@ -45,8 +48,8 @@ __global__ void life_step(const T* __restrict__ cur, T* __restrict__ nxt, int ro
 
 // ---- temporal blocking in LDS, bit-packed ----
 constexpr int LT_THREADS = 256;
-constexpr int LT_TW = 2;     // tile width in 64-cell words
-constexpr int LT_TR = 32;    // tile height in rows
+constexpr int LT_TW = 1;     // tile width in 64-cell words
+constexpr int LT_TR = 16;    // tile height in rows
 constexpr int LT_K = 32;     // steps per launch (<= 64: the side halo is one word)
 
 // bit-sliced full adder helpers on 64-cell words
@@ -143,14 +146,14 @@ __device__ __forceinline__ void unpack64(uint64_t v, T* __restrict__ row)
 // sees its true toroidal neighbours inside the region).
 template <typename T>
 __global__ __launch_bounds__(LT_THREADS) void life_tiles(const T* __restrict__ cur, T* __restrict__ out, int rows,
-                                                         int cols, int k)
+                                                         int cols, int k, int tw, int tr)
 {
 	extern __shared__ uint64_t lds[];
 	const int wpr = cols / 64;
-	const int tiles_x = (wpr + LT_TW - 1) / LT_TW;
+	const int tiles_x = (wpr + tw - 1) / tw;
 	const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-	const int RW = LT_TW + 2, RH = LT_TR + 2 * k;
-	const int q0 = tx * LT_TW - 1, i0 = ty * LT_TR - k;
+	const int RW = tw + 2, RH = tr + 2 * k;
+	const int q0 = tx * tw - 1, i0 = ty * tr - k;
 	const int words = RW * RH;
 	uint64_t* A = lds;
 	uint64_t* B = lds + words;
@@ -180,9 +183,9 @@ __global__ __launch_bounds__(LT_THREADS) void life_tiles(const T* __restrict__ c
 		A = B;
 		B = t;
 	}
-	for (int w = threadIdx.x; w < LT_TW * LT_TR; w += LT_THREADS) {
-		const int ti = w / LT_TW, tq = w % LT_TW;
-		const int i = ty * LT_TR + ti, q = tx * LT_TW + tq;
+	for (int w = threadIdx.x; w < tw * tr; w += LT_THREADS) {
+		const int ti = w / tw, tq = w % tw;
+		const int i = ty * tr + ti, q = tx * tw + tq;
 		if (i >= rows || q >= wpr) continue;
 		unpack64(A[(k + ti) * RW + 1 + tq], out + (size_t)i * cols + (size_t)q * 64);
 	}
@@ -234,18 +237,25 @@ struct LifeProgram : Program {
 		n -= 1;
 		const int rows = m->pub.rows, cols = m->pub.columns;
 		const int wpr = cols / 64;
-		const int blocks = ((wpr + LT_TW - 1) / LT_TW) * ((rows + LT_TR - 1) / LT_TR);
+		int tw = LT_TW, tr = LT_TR, kmax = LT_K;
+		if (const char* e = getenv("CALHIP_LIFE_TILE"))   // "words,rows,steps"
+			if (sscanf(e, "%d,%d,%d", &tw, &tr, &kmax) != 3 || tw < 1 || tr < 1 || kmax < 1 || kmax > 64) {
+				tw = LT_TW;
+				tr = LT_TR;
+				kmax = LT_K;
+			}
+		const int blocks = ((wpr + tw - 1) / tw) * ((rows + tr - 1) / tr);
 		while (n > 0) {
-			const int k = n < LT_K ? n : LT_K;
-			const size_t lds = (size_t)2 * (LT_TW + 2) * (LT_TR + 2 * k) * sizeof(uint64_t);
+			const int k = n < kmax ? n : kmax;
+			const size_t lds = (size_t)2 * (tw + 2) * (tr + 2 * k) * sizeof(uint64_t);
 			hipEvent_t ev;
 			timer_begin(m, &ev);
 			if (q->type == 'b')
 				hipLaunchKernelGGL(life_tiles<int8_t>, dim3(blocks), dim3(LT_THREADS), lds, m->stream,
-				                   (const int8_t*)q->d_cur, (int8_t*)q->d_nxt, rows, cols, k);
+				                   (const int8_t*)q->d_cur, (int8_t*)q->d_nxt, rows, cols, k, tw, tr);
 			else
 				hipLaunchKernelGGL(life_tiles<int32_t>, dim3(blocks), dim3(LT_THREADS), lds, m->stream,
-				                   (const int32_t*)q->d_cur, (int32_t*)q->d_nxt, rows, cols, k);
+				                   (const int32_t*)q->d_cur, (int32_t*)q->d_nxt, rows, cols, k, tw, tr);
 			timer_end(m, ev);
 			m->stats.launches++;
 			if (CALHIP_CHECK(hipGetLastError())) return -1;
