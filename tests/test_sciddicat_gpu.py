@@ -145,6 +145,50 @@ def test_active_round_trips(split):
     m.close()
 
 
+@pytest.mark.parametrize("edit", ["dry_far_cell", "already_active_cell"])
+def test_active_host_edit_mid_run(edit):
+    """The host adds a cell to the active set between steps (calAddActiveCell2D
+    on the host mirror).  A dry cell far from the flow (the first case) is
+    visited by one step's processes, changes nothing and is removed again; a
+    cell already in the set (the second) changes nothing.  Either way the run
+    continues on the device from the re-uploaded state (the first case through
+    the list path for one step, then the tile lists rebuilt from the flags)
+    and ends with the oracle's unedited result and set size."""
+    import ctypes as C
+    from opencal_amd import lib
+    from opencal_amd.models import SciddicaT
+    rows, cols, steps, k = 90, 200, 70, 25
+    z, h, (r0, c0, p) = synth.sciddicat_inputs(rows, cols, 45, patch=10)
+    _, oh, counts = native.oracle_sciddicat(z, h, 0.5, 0.001, steps, True, counts=True)
+    _, oh_k = native.oracle_sciddicat(z, h, 0.5, 0.001, k, True)
+    if edit == "dry_far_cell":
+        # the cell farthest from the source patch whose 3 x 3 block stays dry
+        ii, jj = np.meshgrid(np.arange(rows), np.arange(cols), indexing="ij")
+        d = np.abs(ii - (r0 + p // 2)) + np.minimum(np.abs(jj - (c0 + p // 2)), cols - np.abs(jj - (c0 + p // 2)))
+        dry = np.ones_like(oh, bool)
+        for di in (-1, 0, 1):
+            for dj in (-1, 0, 1):
+                dry &= np.roll(np.roll(oh, di, 0), dj, 1) == 0.0
+        i, j = np.unravel_index(np.argmax(np.where(dry, d, -1)), d.shape)
+        assert dry[i, j]
+    else:
+        wet = np.argwhere(oh_k > 0.001)
+        i, j = wet[len(wet) // 2]
+    print(f"case host edit {edit} at ({i}, {j}) after {k} steps", flush=True)
+    L = lib()
+    L.calAddActiveCell2D.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    m = SciddicaT(z, h, 0.5, 0.001, steps, True)
+    m.init()
+    m.steps(k)
+    L.calAddActiveCell2D(m.model, int(i), int(j))
+    m.steps(steps - k)
+    _, gh, cnt = m.read()
+    assert np.array_equal(gh, oh), f"max |dh| = {np.abs(gh - oh).max()}"
+    assert cnt == counts[-1]
+    assert m.stats()["uploads"] >= 2
+    m.close()
+
+
 def test_mass_conservation_4096():
     """At the benchmark size: total thickness is conserved on the torus (to
     rounding), and the first 3 steps match the oracle bit for bit."""
