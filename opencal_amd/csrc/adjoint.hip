@@ -15,9 +15,11 @@
 //   dL/dz(c) += [wet c, 0 kept] B(c) + the same neighbour terms
 //   dL/dr   += sum_x sum_{n in S(x)} g_n(x) (avg(x) - u_n(x))
 //   dL/deps += sum_x B(x) (-1 + [0 kept in x])
-// The kernel gathers these per cell (one thread per cell; neighbour values come
-// through the caches).  dL/dr and dL/deps are summed per block across all
-// steps (each block only ever adds into its own slot, so the sums are
+// Two kernels per step, one thread per cell: adjoint_local computes each wet
+// cell's B and its dL/dr, dL/deps terms once; adjoint_gather then forms
+// dL/dh and dL/dz from the cell's and its neighbours' B (neighbour values
+// come through the caches).  dL/dr and dL/deps are summed per block across
+// all steps (each block only ever adds into its own slot, so the sums are
 // deterministic) and reduced on the host at the end.
 //
 // Checkpointing: the forward writes h every `ck` steps; the backward walks the
@@ -64,12 +66,21 @@ __device__ __forceinline__ void cell_state(const double* __restrict__ z, const d
 	           s);
 }
 
-__global__ __launch_bounds__(ADJ_BLOCK) void adjoint_step(const double* __restrict__ z,
-                                                          const double* __restrict__ h,
-                                                          const double* __restrict__ lam1,
-                                                          double* __restrict__ lam0, double* __restrict__ lam_z,
-                                                          double* __restrict__ partial, int rows, int cols,
-                                                          double r, double eps, Box box)
+// The adjoint of one step in two launches, so that each wet cell runs its
+// minimisation once instead of once for itself and once for each of its four
+// neighbours.  Both give the same bits as the single-pass gather: every sum
+// below is formed in the same order.
+//
+// adjoint_local, per cell x: the forward state of x; when x is wet,
+// B(x) = r sum_{n in S(x)} (lam(nb_n(x)) - lam(x)) / cnt(x), x's terms of
+// dL/dr and dL/deps (per-block partial sums, deterministic), and
+// info(x) = bit 0 wet, bit n (1..4) neighbour n kept, bit 5 level 0 kept.
+__global__ __launch_bounds__(ADJ_BLOCK) void adjoint_local(const double* __restrict__ z,
+                                                           const double* __restrict__ h,
+                                                           const double* __restrict__ lam1,
+                                                           double* __restrict__ Bx, uint8_t* __restrict__ info,
+                                                           double* __restrict__ partial, int rows, int cols,
+                                                           double r, double eps)
 {
 	__shared__ double red_r[ADJ_BLOCK / 64], red_e[ADJ_BLOCK / 64];
 	const size_t n = (size_t)rows * cols;
@@ -80,9 +91,10 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_step(const double* __restri
 		FlowState s;
 		size_t nb[5];
 		cell_state(z, h, rows, cols, i, j, eps, s, nb);
-		const double lc = lam1[c];
-		double acc = lc, lz = 0.0;
+		double B = 0.0;
+		unsigned inf = 0u;
 		if (s.wet) {
+			const double lc = lam1[c];
 			double sg = 0.0;
 #pragma unroll
 			for (int k = 1; k < 5; ++k)
@@ -93,34 +105,13 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_step(const double* __restri
 				}
 			// cnt == 0: rounding eliminated every level (h - eps below the
 			// resolution of the level sum); no flow depends on the average
-			const double B = s.cnt ? r * sg / s.cnt : 0.0;
-			acc += B;
+			B = s.cnt ? r * sg / s.cnt : 0.0;
 			const bool kept0 = !(s.elim & 1u);
-			if (kept0) lz += B;
 			de += kept0 ? 0.0 : -B;
+			inf = 1u | ((~s.elim & 0x1eu)) | (kept0 ? 32u : 0u);
 		}
-		// neighbour x = nb[k] sending towards c: c is x's neighbour opp(k)
-#pragma unroll
-		for (int k = 1; k < 5; ++k) {
-			const int opp = 5 - k;
-			const int xi = k == 1 ? dev::wrap1(i - 1, rows) : (k == 4 ? dev::wrap1(i + 1, rows) : i);
-			const int xj = k == 2 ? dev::wrap1(j - 1, cols) : (k == 3 ? dev::wrap1(j + 1, cols) : j);
-			FlowState sx;
-			size_t nbx[5];
-			cell_state(z, h, rows, cols, xi, xj, eps, sx, nbx);
-			if (!sx.wet || ((sx.elim >> opp) & 1u)) continue;
-			const double lx = lam1[nbx[0]];
-			double sg = 0.0;
-#pragma unroll
-			for (int q = 1; q < 5; ++q)
-				if (!((sx.elim >> q) & 1u)) sg += lam1[nbx[q]] - lx;
-			const double Bx = sx.cnt ? r * sg / sx.cnt : 0.0;
-			const double du = -r * (lc - lx) + Bx;
-			acc += du;
-			lz += du;
-		}
-		lam0[c] = acc;
-		if (i >= box.r0 && i < box.r0 + box.nr && j >= box.c0 && j < box.c0 + box.nc) lam_z[c] += lz;
+		Bx[c] = B;
+		info[c] = (uint8_t)inf;
 	}
 	for (int off = 32; off > 0; off >>= 1) {
 		dr += __shfl_down(dr, off, 64);
@@ -142,14 +133,52 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_step(const double* __restri
 	}
 }
 
+// adjoint_gather, per cell c: dL/dh(c) = lam(c) + [wet c] B(c)
+// + sum_n [wet x, opp(n) kept in x] (-r (lam(c) - lam(x)) + B(x)), x = nb_n(c),
+// and the same terms (B(c) only when c's level 0 is kept) into dL/dz.
+__global__ __launch_bounds__(ADJ_BLOCK) void adjoint_gather(const double* __restrict__ lam1,
+                                                            const double* __restrict__ Bx,
+                                                            const uint8_t* __restrict__ info,
+                                                            double* __restrict__ lam0, double* __restrict__ lam_z,
+                                                            int rows, int cols, double r, Box box)
+{
+	const size_t n = (size_t)rows * cols;
+	const size_t c = (size_t)blockIdx.x * ADJ_BLOCK + threadIdx.x;
+	if (c >= n) return;
+	const int i = (int)(c / cols), j = (int)(c % cols);
+	const double lc = lam1[c];
+	double acc = lc, lz = 0.0;
+	const unsigned ic = info[c];
+	if (ic & 1u) {
+		const double B = Bx[c];
+		acc += B;
+		if (ic & 32u) lz += B;
+	}
+	// neighbour x = nb[k] sending towards c: c is x's neighbour opp(k)
+#pragma unroll
+	for (int k = 1; k < 5; ++k) {
+		const int opp = 5 - k;
+		const int xi = k == 1 ? dev::wrap1(i - 1, rows) : (k == 4 ? dev::wrap1(i + 1, rows) : i);
+		const int xj = k == 2 ? dev::wrap1(j - 1, cols) : (k == 3 ? dev::wrap1(j + 1, cols) : j);
+		const size_t x = (size_t)xi * cols + xj;
+		const unsigned ix = info[x];
+		if (!(ix & 1u) || !((ix >> opp) & 1u)) continue;
+		const double du = -r * (lc - lam1[x]) + Bx[x];
+		acc += du;
+		lz += du;
+	}
+	lam0[c] = acc;
+	if (i >= box.r0 && i < box.r0 + box.nr && j >= box.c0 && j < box.c0 + box.nc) lam_z[c] += lz;
+}
+
 // Work buffers kept for the life of the process and grown on demand, so that
 // repeated forward / gradient calls (an optimisation loop) do not pay for
 // device allocation every time.  Calls are serialised by the lock they hold
 // for their whole duration.
 struct WorkCache {
 	std::mutex mu;
-	void* p[5] = {};
-	size_t cap[5] = {};
+	void* p[7] = {};
+	size_t cap[7] = {};
 	sct::WetOrder wo;   // wet-first order of the forward sweeps
 	void* get(int slot, size_t bytes)
 	{
@@ -167,7 +196,7 @@ WorkCache& work()
 	static WorkCache* w = new WorkCache();   // never destroyed: freed with the process
 	return *w;
 }
-enum { W_A, W_B, W_TRASH, W_STASH, W_PART };
+enum { W_A, W_B, W_TRASH, W_STASH, W_PART, W_BX, W_INFO };
 
 bool check_args(int rows, int cols, int steps, int ck)
 {
@@ -251,7 +280,9 @@ static int adjoint_impl(int rows, int cols, const double* d_z, const double* d_c
 		double* lamB = (double*)W.get(W_A, bytes);
 		double* partp = (double*)W.get(W_PART, sizeof(double) * 2 * (size_t)nblk);
 		double* trash = (double*)W.get(W_TRASH, 128 * sizeof(double));
-		if (!stash || !lamB || !partp || !trash) break;
+		double* Bx = (double*)W.get(W_BX, bytes);
+		uint8_t* info = (uint8_t*)W.get(W_INFO, n);
+		if (!stash || !lamB || !partp || !trash || !Bx || !info) break;
 		if (CALHIP_CHECK(hipMemsetAsync(partp, 0, sizeof(double) * 2 * (size_t)nblk, st))) break;
 		if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
 		if (CALHIP_CHECK(hipEventRecord(e0, st))) break;
@@ -269,9 +300,12 @@ static int adjoint_impl(int rows, int cols, const double* d_z, const double* d_c
 				launch_sweep(st, rows, cols, d_z, S + (size_t)(t - t0) * n, S + (size_t)(t - t0 + 1) * n, trash, p_r,
 				             p_eps, shape, &W.wo);
 			for (int t = t1 - 1; t >= t0; --t) {
-				hipLaunchKernelGGL(adjoint_step, dim3(nblk), dim3(ADJ_BLOCK), 0, st, d_z,
-				                   (const double*)(S + (size_t)(t - t0) * n), (const double*)lam[cur], lam[cur ^ 1],
-				                   d_lam_z, partp, rows, cols, p_r, p_eps, box);
+				hipLaunchKernelGGL(adjoint_local, dim3(nblk), dim3(ADJ_BLOCK), 0, st, d_z,
+				                   (const double*)(S + (size_t)(t - t0) * n), (const double*)lam[cur], Bx, info,
+				                   partp, rows, cols, p_r, p_eps);
+				hipLaunchKernelGGL(adjoint_gather, dim3(nblk), dim3(ADJ_BLOCK), 0, st, (const double*)lam[cur],
+				                   (const double*)Bx, (const uint8_t*)info, lam[cur ^ 1], d_lam_z, rows, cols, p_r,
+				                   box);
 				cur ^= 1;
 			}
 		}
