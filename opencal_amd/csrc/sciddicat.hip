@@ -173,10 +173,12 @@ __global__ void zero_at_active(FlowPtrs f, const int* __restrict__ cells, const 
 // clears again.  The grid is cut into tiles of 4 rows x 64 columns (one wave
 // per row segment) and each launch visits only listed tiles.  For step epoch
 // e (a counter the host passes, one per step):
-//   * S[p]: the tiles that hold cells of the set, each with in_set[t] == e
+//   * S[p]: the tiles that hold cells of the set, each tagged tag[t] == 2e
 //     (built from the flags when the step starts from a host-visible state,
 //     then by the width process of the step before);
-//   * X[p]: the other tiles that hold receivers, each once (tmark[t] == e).
+//   * X[p]: the other tiles that hold receivers, each once: a tile joins
+//     when its atomic max with 2e + 1 returns a tag below 2e (an older
+//     epoch), so a tile of S[p] or one already listed is not listed again.
 // The width process visits S[p] and X[p].  No launch depends on a host
 // value and no cell list is rebuilt between the processes; the row-major
 // list is rebuilt only when it is observed.  Every cell processed is one the
@@ -188,8 +190,7 @@ struct TileLists {
 	int* ctr;                // [4]
 	int* set_list[2];        // [n_tiles] each
 	int* x_list[2];          // [n_tiles] each
-	unsigned* in_set;        // [n_tiles] epoch of the set list that holds the tile
-	unsigned* tmark;         // [n_tiles] epoch of the receiver list that holds it
+	unsigned* tag;           // [n_tiles] 2e: in S of epoch e; 2e + 1: reached in epoch e
 	uint8_t* recv;           // [cells]
 	int ty, tx, n_tiles;     // tile rows, tile columns, tiles
 };
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void tiles_from_flags(const uint8_t* __restric
 		const bool in = i < rows && j < cols && flags[(size_t)i * cols + j] != 0;
 		if (__syncthreads_or((int)in) && threadIdx.x == 0) {
 			tl.set_list[p][atomicAdd(tl.ctr + p, 1)] = t;
-			tl.in_set[t] = e;
+			tl.tag[t] = 2u * e;
 		}
 	}
 }
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(256) void flows_tiles(const double* __restrict__ z,
 			if (threadIdx.x == 1) u = tr + ((tc + tl.tx - 1) % tl.tx) * tl.ty;
 			if (threadIdx.x == 2) u = tr + ((tc + 1) % tl.tx) * tl.ty;
 			if (threadIdx.x == 3) u = (tr + 1) % tl.ty + tc * tl.ty;
-			if (tl.in_set[u] != e && atomicExch(tl.tmark + u, e) != e) tl.x_list[p][atomicAdd(tl.ctr + 2 + p, 1)] = u;
+			if (atomicMax(tl.tag + u, 2u * e + 1u) < 2u * e) tl.x_list[p][atomicAdd(tl.ctr + 2 + p, 1)] = u;
 		}
 	}
 }
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(256) void flows_tiles(const double* __restrict__ z,
 // cell while both are empty), in place, fused with the removal process: a
 // visited cell stays in the set (flag 1) when it was in it or received and
 // not h' <= eps; recv is cleared.  The tiles keeping cells of the set go to
-// S[1-p] with in_set = e + 1.  Clears |X[1-p]| for the next flows process.
+// S[1-p] tagged 2(e + 1).  Clears |X[1-p]| for the next flows process.
 __global__ __launch_bounds__(256) void width_tiles(double* __restrict__ h, FlowPtrs f, uint8_t* __restrict__ flags,
                                                    TileLists tl, int p, unsigned e, int rows, int cols, double eps)
 {
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(256) void width_tiles(double* __restrict__ h, FlowP
 		}
 		if (__syncthreads_or((int)kept) && threadIdx.x == 0) {
 			tl.set_list[1 - p][atomicAdd(tl.ctr + 1 - p, 1)] = t;
-			tl.in_set[t] = e + 1u;
+			tl.tag[t] = 2u * (e + 1u);
 		}
 	}
 }
@@ -424,7 +425,7 @@ struct SciddicaTProgram : Program {
 	// flag-driven active step: tile lists by parity (one allocation from
 	// tiles.ctr); the row-major list is rebuilt on demand
 	TileLists tiles = {};
-	unsigned tile_epoch = 0;     // step epoch of the tile lists (wraps after 2^32 steps)
+	unsigned tile_epoch = 0;     // step epoch of the tile lists (restarts before 2^31)
 	int parity = 0;
 	bool ctr_valid = false;      // tiles.set_list[parity] lists the set's tiles
 	bool list_stale = false;     // m->d_cells / d_count lag behind the flags
@@ -537,7 +538,7 @@ struct SciddicaTProgram : Program {
 			tiles.tx = (cols + 63) / 64;
 			tiles.n_tiles = tiles.ty * tiles.tx;
 			const size_t nt = (size_t)tiles.n_tiles;
-			const size_t bytes = 4 * sizeof(int) + 4 * nt * sizeof(int) + 2 * nt * sizeof(unsigned) + m->cells();
+			const size_t bytes = 4 * sizeof(int) + 4 * nt * sizeof(int) + nt * sizeof(unsigned) + m->cells();
 			char* b = nullptr;
 			if (CALHIP_CHECK(hipMalloc(&b, bytes))) return -1;
 			if (CALHIP_CHECK(hipMemsetAsync(b, 0, bytes, m->stream))) {
@@ -550,24 +551,22 @@ struct SciddicaTProgram : Program {
 			tiles.set_list[1] = lists + nt;
 			tiles.x_list[0] = lists + 2 * nt;
 			tiles.x_list[1] = lists + 3 * nt;
-			tiles.in_set = reinterpret_cast<unsigned*>(lists + 4 * nt);
-			tiles.tmark = tiles.in_set + nt;
-			tiles.recv = reinterpret_cast<uint8_t*>(tiles.tmark + nt);
+			tiles.tag = reinterpret_cast<unsigned*>(lists + 4 * nt);
+			tiles.recv = reinterpret_cast<uint8_t*>(tiles.tag + nt);
 			ctr_valid = false;
 		}
 		const int grid = std::min(tiles.n_tiles, 1024);
 		hipEvent_t ev;
 		timer_begin(m, &ev);
-		if (tile_epoch >= 0xfffffff0u) {
-			// the epochs wrap: start the tags again from the arrays' zero state
-			if (CALHIP_CHECK(hipMemsetAsync(tiles.in_set, 0, 2 * (size_t)tiles.n_tiles * sizeof(unsigned),
-			                                m->stream)))
+		if (tile_epoch >= 0x7ffffff0u) {
+			// the tags (2e + 1) would wrap: start again from the zero state
+			if (CALHIP_CHECK(hipMemsetAsync(tiles.tag, 0, (size_t)tiles.n_tiles * sizeof(unsigned), m->stream)))
 				return -1;
 			tile_epoch = 0;
 			ctr_valid = false;
 		}
 		// a rebuilt list needs an epoch no tag holds yet: the last width
-		// process tagged its tiles with (last epoch + 1)
+		// process tagged its tiles for (last epoch + 1)
 		if (!ctr_valid) tile_epoch++;
 		const unsigned e = ++tile_epoch;
 		if (!ctr_valid) {
