@@ -10,29 +10,35 @@
 //   it (:94-100).
 //
 // Kernels:
-//   step_fused2   full grid, both processes in one sweep (sweep_strip in
-//                 sciddicat_device.hpp).  Each wave owns a 128-column strip
-//                 (two cells per lane, 124 output columns plus a 2-column halo
-//                 on each side) and walks a chunk of 8 rows with a register
-//                 ring of z and h rows; horizontal neighbours come from the
-//                 adjacent lanes.  The outflows of a cell are computed once,
-//                 consumed by the width update of its four neighbours through
-//                 lane shuffles (east/west) and the ring (north/south), and
-//                 never written to memory.  Neighbouring chunks walk in
-//                 opposite directions so the halo rows they share are read
-//                 close together in time (from L2), and blocks are remapped so
-//                 that each XCD sweeps one contiguous band of the grid.  HBM
-//                 traffic per cell: z and h read, h written = 24 B.
+//   step_fused_al, step_fused2 (sciddicat_device.hpp)
+//                 full grid, both processes in one sweep.  Each wave owns a
+//                 strip (128 aligned columns for even column counts, 124 plus
+//                 a 2-column halo for odd ones), two cells per lane, and walks
+//                 a chunk of rows with a register ring of z and h rows;
+//                 horizontal neighbours come from the adjacent lanes.  The
+//                 outflows of a cell are computed once, consumed by the width
+//                 update of its four neighbours through lane shuffles and the
+//                 ring, and never written to memory.  HBM traffic per cell: z
+//                 and h read, h written = 24 B.
+//   step_fused_2x (sweep_tune.hip, strip sweep in sciddicat_2step.hpp)
+//                 two steps per sweep when the steering is declared to be the
+//                 reference's flow reset (calhipBindSciddicaTSteering2D).
 //   flows_full / width_full
 //                 one process per sweep with the flows written out, the exact
 //                 reference sequence for models whose flows are observed.
+//   tiles_from_flags, flows_tiles, width_tiles
+//                 the active-cells step on the flag grid over lists of 4 x 64
+//                 cell tiles (the set's tiles and the receivers' tiles).
 //   flows_active / width_active
 //                 over the row-major active list (or the whole grid when it is
-//                 empty, as OpenCAL does when A.cells == NULL).
+//                 empty, as OpenCAL does when A.cells == NULL), for a step
+//                 that starts after the host changed the set.
 //
 // Arithmetic follows the reference operation by operation (no contraction:
 // the library is built with -ffp-contract=off), so results are bit-identical
-// to the CPU path.
+// to the CPU path; the division of the level sum by the level count is
+// computed with FMAs in a form that gives the IEEE quotient's bits
+// (div_count, sciddicat_device.hpp).
 #include <stdio.h>
 
 #include <algorithm>
