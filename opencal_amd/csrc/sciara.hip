@@ -213,21 +213,37 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width(double* __restrict__ z, dou
 	const size_t n = (size_t)rows * cols;
 	const int cnt = ACTIVE ? count[0] + count[1] : 0;
 	const size_t total = sweep_total(cnt, n);
-	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-	     idx += (size_t)gridDim.x * blockDim.x) {
+	// eight lanes per cell, as in sc_flows: lane k-1 of a group loads
+	// neighbour k's inflow, temperature and the cell's outflow towards k;
+	// every lane then sums the group's eight terms in k order
+	const int lane = (int)(threadIdx.x & 63);
+	const int g0 = lane & ~7;
+	const int k = (lane & 7) + 1;
+	const bool writer = (lane & 7) == 0;
+	const double* fin = f.f[opposite[0]];
+	const double* fout = f.f[0];
+#pragma unroll
+	for (int q = 1; q < NF; ++q)
+		if (k - 1 == q) {
+			fin = f.f[opposite[q]];
+			fout = f.f[q];
+		}
+	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
+	for (size_t idx = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3); idx < total; idx += stride) {
 		const int c = cnt > 0 ? cells[idx] : (int)idx;
 		const int i = c / cols, j = c - (c / cols) * cols;
 		const double ih = h[c], it = T[c];
+		const int qk = nb_flat(i, j, k, rows, cols);
+		const double in_k = qk >= 0 ? fin[qk] : 0.0;
+		const double out_k = fout[c];
+		const double nt_k = qk >= 0 ? T[qk] : 0.0;
+		const double ht_k = in_k * nt_k;
 		double ht = 0.0, in_sum = 0.0, out_sum = 0.0;
 #pragma unroll
-		for (int k = 1; k < 9; ++k) {
-			const int q = nb_flat(i, j, k, rows, cols);
-			const double in_flow = q >= 0 ? f.f[opposite[k - 1]][q] : 0.0;
-			const double out_flow = f.f[k - 1][c];
-			const double nt = q >= 0 ? T[q] : 0.0;
-			ht += in_flow * nt;
-			in_sum += in_flow;
-			out_sum += out_flow;
+		for (int m = 0; m < 8; ++m) {
+			ht += __shfl(ht_k, g0 + m, 64);
+			in_sum += __shfl(in_k, g0 + m, 64);
+			out_sum += __shfl(out_k, g0 + m, 64);
 		}
 		double h1 = ih;
 		h1 += in_sum - out_sum;
@@ -247,15 +263,19 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width(double* __restrict__ z, dou
 			const double st = nT / pm_cbrt_pos(aus);
 			t2 = st;
 			if (st <= P.PTsol) {
-				z[c] = z[c] + h1;
-				Msl[c] = Msl[c] + h1;
+				if (writer) {
+					z[c] = z[c] + h1;
+					Msl[c] = Msl[c] + h1;
+				}
 				h2 = 0.0;
 				t2 = P.PTsol;
 			}
 		}
-		h[c] = h2;
-		Tn[c] = t2;
-		if (ACTIVE && t2 <= P.PTsol && !mb) flags[c] = 0;
+		if (writer) {
+			h[c] = h2;
+			Tn[c] = t2;
+			if (ACTIVE && t2 <= P.PTsol && !mb) flags[c] = 0;
+		}
 	}
 }
 
@@ -588,11 +608,11 @@ struct SciaraProgram : Program {
 			                   rows, cols, P, 0);
 		double* Tn = (double*)T->d_nxt;
 		if (active)
-			hipLaunchKernelGGL(sc_width<true>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (double*)z->d_cur,
+			hipLaunchKernelGGL(sc_width<true>, dim3(grid8), dim3(SC_BLOCK), 0, m->stream, (double*)z->d_cur,
 			                   (double*)h->d_cur, (const double*)T->d_cur, Tn, F, (const uint8_t*)mb->d_cur,
 			                   (double*)msl->d_cur, m->d_flags, m->d_cells, m->d_count, rows, cols, P);
 		else
-			hipLaunchKernelGGL(sc_width<false>, dim3(grid), dim3(SC_BLOCK), 0, m->stream, (double*)z->d_cur,
+			hipLaunchKernelGGL(sc_width<false>, dim3(grid8), dim3(SC_BLOCK), 0, m->stream, (double*)z->d_cur,
 			                   (double*)h->d_cur, (const double*)T->d_cur, Tn, F, (const uint8_t*)mb->d_cur,
 			                   (double*)msl->d_cur, nullptr, nullptr, nullptr, rows, cols, P);
 		if (active) {
