@@ -232,10 +232,13 @@ __global__ __launch_bounds__(256) void flows_tiles(const double* __restrict__ z,
 	__shared__ unsigned sh_dirs;
 	if (blockIdx.x == 0 && threadIdx.x == 0) tl.ctr[1 - p] = 0;
 	const int n_set = tl.ctr[p];
+	// this block's first list entry, loaded beside the count instead of after
+	// it (in bounds: gridDim.x <= n_tiles); used only when the count covers it
+	const int t_first = tl.set_list[p][blockIdx.x];
 	const bool all = n_set == 0;
 	const int n_iter = all ? tl.n_tiles : n_set;
 	for (int k = blockIdx.x; k < n_iter; k += gridDim.x) {
-		const int t = all ? k : tl.set_list[p][k];
+		const int t = all ? k : (k == (int)blockIdx.x ? t_first : tl.set_list[p][k]);
 		if (threadIdx.x == 0) sh_dirs = 0u;
 		__syncthreads();
 		const int i = (t % tl.ty) * 4 + (int)(threadIdx.x >> 6);
@@ -296,10 +299,15 @@ __global__ __launch_bounds__(256) void width_tiles(double* __restrict__ h, FlowP
 {
 	if (blockIdx.x == 0 && threadIdx.x == 0) tl.ctr[3 - p] = 0;
 	const int n_set = tl.ctr[p], n_x = tl.ctr[2 + p];
+	// Block b takes the set's entries b, b + G, ... and the receivers' entries
+	// G-1-b, 2G-1-b, ... (G = gridDim.x <= n_tiles).  Its first entry of each
+	// list is loaded beside the counts instead of after them and used only
+	// when the count covers it.
+	const int G = (int)gridDim.x, b = (int)blockIdx.x;
+	const int s_first = tl.set_list[p][b];
+	const int x_first = tl.x_list[p][G - 1 - b];
 	const bool all = n_set == 0 && n_x == 0;
-	const int n_iter = all ? tl.n_tiles : n_set + n_x;
-	for (int k = blockIdx.x; k < n_iter; k += gridDim.x) {
-		const int t = all ? k : (k < n_set ? tl.set_list[p][k] : tl.x_list[p][k - n_set]);
+	auto visit = [&](int t) {
 		const int i = (t % tl.ty) * 4 + (int)(threadIdx.x >> 6);
 		const int j = (t / tl.ty) * 64 + (int)(threadIdx.x & 63);
 		bool kept = false;
@@ -327,7 +335,13 @@ __global__ __launch_bounds__(256) void width_tiles(double* __restrict__ h, FlowP
 			tl.set_list[1 - p][atomicAdd(tl.ctr + 1 - p, 1)] = t;
 			tl.tag[t] = 2u * (e + 1u);
 		}
+	};
+	if (all) {
+		for (int k = b; k < tl.n_tiles; k += G) visit(k);
+		return;
 	}
+	for (int k = b; k < n_set; k += G) visit(k == b ? s_first : tl.set_list[p][k]);
+	for (int m = G - 1 - b; m < n_x; m += G) visit(m == G - 1 - b ? x_first : tl.x_list[p][m]);
 }
 
 int stride_grid(size_t n, int block)
