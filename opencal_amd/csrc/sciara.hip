@@ -144,8 +144,12 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 	const int g0 = lane & ~7;          // first lane of this lane's group
 	const int k = (lane & 7) + 1;      // the neighbour this lane handles
 	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
-	for (size_t idx = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3); idx < total; idx += stride) {
-		const int c = cnt0 > 0 ? cells[idx] : (int)idx;
+	const size_t idx0 = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3);
+	// the first list entry, loaded beside the count (clamped into the list's
+	// n entries; used only when the count covers it)
+	const int c_first = ACTIVE ? cells[min(idx0, n - 1)] : 0;
+	for (size_t idx = idx0; idx < total; idx += stride) {
+		const int c = cnt0 > 0 ? (idx == idx0 ? c_first : cells[idx]) : (int)idx;
 		const double h0 = h[c];
 		if (!(h0 > 0)) continue;
 		const int i = c / cols, j = c - (c / cols) * cols;
@@ -229,8 +233,10 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width(double* __restrict__ z, dou
 			fout = f.f[q];
 		}
 	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
-	for (size_t idx = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3); idx < total; idx += stride) {
-		const int c = cnt > 0 ? cells[idx] : (int)idx;
+	const size_t idx0 = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3);
+	const int c_first = ACTIVE ? cells[min(idx0, n - 1)] : 0;   // as in sc_flows
+	for (size_t idx = idx0; idx < total; idx += stride) {
+		const int c = cnt > 0 ? (idx == idx0 ? c_first : cells[idx]) : (int)idx;
 		const int i = c / cols, j = c - (c / cols) * cols;
 		const double ih = h[c], it = T[c];
 		const int qk = nb_flat(i, j, k, rows, cols);
@@ -306,10 +312,12 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_finish(double* __restrict__ h, do
 			h[c] = 0.0;
 			T[c] = 0.0;
 		}
+	const size_t idx0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	const int c_first = ACTIVE ? cells[min(idx0, n - 1)] : 0;   // as in sc_flows
 	for (size_t base = (size_t)blockIdx.x * blockDim.x; base < total; base += stride) {
 		const size_t idx = base + threadIdx.x;
 		const bool valid = idx < total;
-		const int c = valid ? (cnt > 0 ? cells[idx] : (int)idx) : 0;
+		const int c = valid ? (cnt > 0 ? (idx == idx0 ? c_first : cells[idx]) : (int)idx) : 0;
 		bool surv = false;
 		if (valid) {
 			const bool mb = Mb[c] != 0;
