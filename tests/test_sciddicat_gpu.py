@@ -189,6 +189,46 @@ def test_active_host_edit_mid_run(edit):
     m.close()
 
 
+@pytest.mark.parametrize("active", [False, True])
+def test_special_values(active):
+    """Infinite altitudes (a +inf wall below the source, a -inf sink beside
+    it) and negative-zero thickness.  A wave with an infinite level takes the
+    minimisation's select-and-add form instead of the FMA form, an infinite
+    level sum meets the division's non-finite case, and rows of -0 that stay
+    dry are written as +0 by the update (the two-step sweep's dry-row shortcut
+    does the same).  Bitwise against the oracle (zero signs and infinities
+    included), through calRun2D and through per-step launches."""
+    from opencal_amd.models import SciddicaT
+    z, h, (r0, c0, p) = synth.sciddicat_inputs(72, 136, 17, patch=12)
+    rows, cols = z.shape
+    z[r0 + p + 2, max(c0 - 10, 0):min(c0 + p + 10, cols)] = np.inf
+    z[r0 + p // 2, (c0 + p + 2) % cols] = -np.inf
+    h[r0 + p + 6:r0 + p + 12, :] = -0.0
+    h[:, (c0 + p + 5) % cols] = -0.0
+    print(f"case special values active={active} patch=({r0}, {c0}, {p})", flush=True)
+
+    def same(a, b):
+        return np.array_equal(a.view(np.int64), b.view(np.int64))
+
+    for steps in (1, 2, 41):
+        gz, gh, gc = gpu_run(z, h, 0.5, 0.001, steps, active)
+        oz, oh, oc = native.oracle_sciddicat(z, h, 0.5, 0.001, steps, active, counts=True)
+        assert same(gz, oz) and same(gh, oh), f"steps={steps}"
+        assert not np.isnan(gh).any()
+        if active:
+            assert gc == oc[-1]
+    m = SciddicaT(z, h, 0.5, 0.001, 41, active)
+    try:
+        m.init()
+        m.steps(41)
+        _, gh, _ = m.read()
+    finally:
+        m.close()
+    _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 41, active)
+    assert same(gh, oh)
+    assert (np.signbit(gh) & (gh == 0.0)).any() == active   # -0 kept where no process visits
+
+
 def test_mass_conservation_4096():
     """At the benchmark size: total thickness is conserved on the torus (to
     rounding), and the first 3 steps match the oracle bit for bit."""
