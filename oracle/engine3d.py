@@ -2,7 +2,7 @@
 reductions, in plain Python/numpy.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ (and nothing else) as the
-checker of the GPU engine's 3D models and reductions; the product never
+oracle of the GPU engine's 3D models and reductions; the product never
 imports, links or runs anything under oracle/.
 
 Follows OpenCAL-OMP:

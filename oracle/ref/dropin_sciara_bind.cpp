@@ -1,7 +1,7 @@
 // The binding a maintainer adds to the reference SCIARA-fv2 program
 // (OpenCAL-OMP/examples/calomp_sciara-fv2/source/main.cpp) to run it on the
 // MI355X engine: called once after loadConfiguration(), when the vents and
-// emission rates are known.  Same code as shown in INTEGRATION.md.  Built only
+// emission rates are known.  Built only
 // into the dropin_sciara program (oracle/ref/Makefile).
 #include "Sciara.h"
 

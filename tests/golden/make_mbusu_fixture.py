@@ -6,7 +6,7 @@ sources on the reference OpenCAL-OMP 3D engine in this container
 
 The reference also ships ris10g-orig-opencal-omp.txt
 (mbusu_ris10g-orig-opencal-omp.txt.gz here), written on another machine;
-this build differs from it by up to 3.3e-5 (its own serial and OpenMP files
+the reference build above differs from it by up to 3.3e-5 (its own serial and OpenMP files
 differ by 2.4e-5), so both are kept.  Runs for a few minutes on 8 cores.
 
     python3 tests/golden/make_mbusu_fixture.py

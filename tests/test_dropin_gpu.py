@@ -1,6 +1,6 @@
 """The reference's own SciddicaT program (OpenCAL-OMP/examples/calomp_sciddicaT/
 source/sciddicaT.c: its main, data paths and output), built against this
-engine with the two-line patch of INTEGRATION.md (oracle/ref/Makefile target
+engine with a two-line patch (oracle/ref/Makefile target
 dropin_sciddicaT), runs on the GPU and writes the reference's shipped
 width_final.txt byte for byte.  Skipped where that build is absent."""
 import gzip
