@@ -47,3 +47,28 @@ def test_batch_zero_steps_returns_input():
     z, h, _ = synth.sciddicat_inputs(40, 50, 3, patch=8)
     outs, _ = sciddicat_batch([(z, h, 0.5, 0.001)], 0)
     assert np.array_equal(outs[0], h)
+
+
+def test_batch_special_values():
+    """Instances with infinite altitudes (a +inf wall below the source, a
+    -inf sink beside it) and rows of negative-zero thickness beside plain
+    instances: the batch's minimisation takes the select-and-add form in the
+    waves that meet an infinite level, and each instance still equals the
+    oracle bit for bit (zero signs and infinities included)."""
+    from opencal_amd.models import sciddicat_batch
+    insts = []
+    for k, (rows, cols) in enumerate([(64, 96), (40, 130), (72, 72), (33, 47)]):
+        z, h, (r0, c0, p) = synth.sciddicat_inputs(rows, cols, 500 + k, patch=10)
+        if k != 3:
+            z[min(r0 + p + 2, rows - 1), max(c0 - 8, 0):min(c0 + p + 8, cols)] = np.inf
+            z[r0 + p // 2, (c0 + p + 2) % cols] = -np.inf
+            h[min(r0 + p + 5, rows - 1), :] = -0.0
+            h[:, (c0 + p + 4) % cols] = -0.0
+        insts.append((z, h, 0.5, 0.001))
+    print("case batch special values", [z.shape for z, *_ in insts], flush=True)
+    for steps in (1, 40):
+        outs, _ = sciddicat_batch(insts, steps)
+        for (z, h, pr, pe), got in zip(insts, outs):
+            _, want = native.oracle_sciddicat(z, h, pr, pe, steps, False)
+            assert np.array_equal(got.view(np.int64), want.view(np.int64)), (z.shape, steps)
+            assert not np.isnan(got).any()
