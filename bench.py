@@ -311,7 +311,7 @@ def bench_sciara(n=2048, seed=3, cpu_steps=200):
     # full-grid active-set rebuilds and barely depends on the lava extent
     nt = native.set_threads(int(os.environ.get("OMP_NUM_THREADS", "16")))
     t0 = time.perf_counter()
-    native.oracle_sciara(inp, cpu_steps, True, portable=False)
+    native.oracle_sciara(inp, cpu_steps, True, crmath=False)
     cw = time.perf_counter() - t0
     out["cpu_port_ms_per_step"] = cw / cpu_steps * 1e3
     out["cpu_sample"] = f"oracle (libm mode) first {cpu_steps} steps incl. init, {nt} threads"

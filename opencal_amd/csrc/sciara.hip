@@ -26,9 +26,10 @@
 //   count[0] list length, count[1] cells appended by sc_flows,
 //   count[2] survivors written by sc_finish, count[3] finished blocks (the
 //   last block moves count[2] to count[0] and clears the others).
-// Elementary functions are the portable ones of portable_math.h (+ - * /
-// only), so the oracle built with the same functions reproduces every bit;
-// against a glibc build they differ by a few ulp per call.
+// Elementary functions are the correctly rounded ones of crmath.h; the
+// oracle built with the same source reproduces every bit, and a glibc build
+// differs only where glibc itself rounds the wrong way (about one call in
+// 10^3 to 10^4, by one ulp).
 #include <math.h>
 #include <string.h>
 
@@ -37,7 +38,7 @@
 
 #include "engine.hpp"
 #include "device_common.hpp"
-#include "portable_math.h"
+#include "crmath.h"
 
 namespace calhip {
 
@@ -154,8 +155,8 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 		if (!(h0 > 0)) continue;
 		const int i = c / cols, j = c - (c / cols) * cols;
 		const double t = T[c];
-		const double Pr = pm_pow10(P.a + P.b * t);
-		const double hc = pm_pow10(P.c + P.d * t);
+		const double Pr = cr_pow10(P.a + P.b * t);
+		const double hc = cr_pow10(P.c + P.d * t);
 		const double sz0 = z[c];
 		const int nbk = nb_flat(i, j, k, rows, cols);
 		const double hk = nbk >= 0 ? h[nbk] : 0.0;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 		const double lk = zk + hk;
 		const bool down = lvl0 > lk;
 		const double Hk = down ? lk : 0.0;
-		const double hcck = hc * pm_cos(pm_atan((lvl0 - lk) / P.Pc));
+		const double hcck = hc * cr_cos(cr_atan((lvl0 - lk) / P.Pc));
 		double H[9];
 		H[0] = sz0;
 #pragma unroll
@@ -263,10 +264,10 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width(double* __restrict__ z, dou
 		const bool mb = Mb[c] != 0;
 		if (h1 > 0 && !mb) {
 			const double nT = t1;
-			const double num = (((3.0 * (nT * nT * nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
+			const double num = (((3.0 * cr_cube(nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
 			const double den = ((P.Prho * P.Pcv) * h1) * P.Pac;
 			const double aus = 1.0 + num / den;
-			const double st = nT / pm_cbrt_pos(aus);
+			const double st = nT / cr_pow_third(aus);
 			t2 = st;
 			if (st <= P.PTsol) {
 				if (writer) {

@@ -31,8 +31,8 @@ int orc_life_run(int rows, int cols, const unsigned char* alive0, int steps,
                  unsigned char* out);
 
 /* SCIARA-fv2 (same argument meaning as the reference harness). math_mode:
- * 0 = host libm (bit-identical to the reference build), 1 = the portable
- * elementary functions shared with the device kernels. */
+ * 0 = host libm (bit-identical to the reference build), 1 = the correctly
+ * rounded elementary functions shared with the device kernels (crmath.h). */
 int orc_sciara_run(int rows, int cols, const double* z0, double cell_size,
                    const double* params, int n_vents, const int* vent_rc,
                    int n_rates, const double* rates, unsigned emission_time,
@@ -40,9 +40,16 @@ int orc_sciara_run(int rows, int cols, const double* z0, double cell_size,
                    double* out_z, double* out_h, double* out_T, double* out_Msl,
                    int* out_steps, int* out_active);
 
-/* The portable elementary functions elementwise (fn: 0 pow10, 1 atan, 2 cos,
- * 3 cbrt, 4 log, 5 exp); returns -1 for an unknown fn. */
-int orc_pm_eval(int fn, const double* x, int n, double* y);
+/* The correctly rounded functions of crmath.h elementwise, their binary128
+ * values rounded to double, and the host libm calls of the reference (fn: 0
+ * pow(10, x), 1 atan, 2 cos, 3 pow(x, 3.0), 4 pow(x, 1.0/3.0)); -1 for an
+ * unknown fn.  orc_cr_check counts mismatches over n seeded points of
+ * [lo, hi] (out: vs binary128, vs libm, max ulp vs libm, max ulp vs
+ * binary128, libm vs binary128). */
+int orc_cr_eval(int fn, const double* x, int n, double* y);
+int orc_quad_eval(int fn, const double* x, int n, double* y);
+int orc_libm_eval(int fn, const double* x, int n, double* y);
+int orc_cr_check(int fn, double lo, double hi, int log_scale, long n, unsigned long long seed, double* out);
 
 /* One SciddicaT full-grid step h -> h_next (flows with the reference's
  * minimisation + width update), used by the adjoint oracle. */

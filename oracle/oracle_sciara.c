@@ -8,8 +8,8 @@
  * Moore, non-toroidal (CAL_SPACE_FLAT) model.
  *
  * math_mode 0 calls the host libm (pow, atan, cos) as the reference does;
- * math_mode 1 calls the portable functions that the device program also uses
- * (opencal_amd/csrc/portable_math.h).
+ * math_mode 1 calls the correctly rounded functions that the device program
+ * also uses (opencal_amd/csrc/crmath.h).
  *
  * Out-of-grid neighbours: the reference reads outside its buffers there (no
  * bounds check on CAL_SPACE_FLAT, cal2D.c:859-870).  Only border cells have
@@ -19,7 +19,7 @@
  * lava volume balance, temperature bounds. */
 #include "oracle.h"
 #include "oracle_engine.h"
-#include "../opencal_amd/csrc/portable_math.h"
+#include "../opencal_amd/csrc/crmath.h"
 #include <math.h>
 #include <omp.h>
 
@@ -41,11 +41,11 @@ typedef struct {
 	int math;
 } sciara_t;
 
-static double m_pow10(const sciara_t* s, double x) { return s->math ? pm_pow10(x) : pow(10, x); }
-static double m_atan(const sciara_t* s, double x) { return s->math ? pm_atan(x) : atan(x); }
-static double m_cos(const sciara_t* s, double x) { return s->math ? pm_cos(x) : cos(x); }
-static double m_cube(const sciara_t* s, double x) { return s->math ? x * x * x : pow(x, 3.0); }
-static double m_cbrt(const sciara_t* s, double x) { return s->math ? pm_cbrt_pos(x) : pow(x, 1.0 / 3.0); }
+static double m_pow10(const sciara_t* s, double x) { return s->math ? cr_pow10(x) : pow(10, x); }
+static double m_atan(const sciara_t* s, double x) { return s->math ? cr_atan(x) : atan(x); }
+static double m_cos(const sciara_t* s, double x) { return s->math ? cr_cos(x) : cos(x); }
+static double m_cube(const sciara_t* s, double x) { return s->math ? cr_cube(x) : pow(x, 3.0); }
+static double m_cbrt(const sciara_t* s, double x) { return s->math ? cr_pow_third(x) : pow(x, 1.0 / 3.0); }
 
 /* neighbour n of (i, j) on the flat grid, -1 outside */
 static inline int nb_flat(const orc_grid* g, int i, int j, int n)
@@ -319,22 +319,5 @@ int orc_sciara_run(int rows, int cols, const double* z0, double cell_size,
 	free(s.Mb);
 	free(s.Msl);
 	orc_grid_free(&s.g);
-	return 0;
-}
-
-/* Elementwise evaluation of the portable functions (fn: 0 pow10, 1 atan,
- * 2 cos, 3 cbrt, 4 log, 5 exp), so tests can bound them against libm. */
-int orc_pm_eval(int fn, const double* x, int n, double* y)
-{
-	int k;
-	for (k = 0; k < n; k++) switch (fn) {
-		case 0: y[k] = pm_pow10(x[k]); break;
-		case 1: y[k] = pm_atan(x[k]); break;
-		case 2: y[k] = pm_cos(x[k]); break;
-		case 3: y[k] = pm_cbrt_pos(x[k]); break;
-		case 4: y[k] = pm_log(x[k]); break;
-		case 5: y[k] = pm_exp(x[k]); break;
-		default: return -1;
-	}
 	return 0;
 }

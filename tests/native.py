@@ -151,9 +151,10 @@ def _sciara_call(fn, inp, steps, active, extra):
     return out
 
 
-def oracle_sciara(inp, steps, active=True, portable=False):
-    """Oracle SCIARA-fv2 run; portable=True uses the shared portable functions."""
-    return _sciara_call(oracle().orc_sciara_run, inp, steps, active, (int(portable),))
+def oracle_sciara(inp, steps, active=True, crmath=False):
+    """Oracle SCIARA-fv2 run: host libm (the reference's arithmetic), or with
+    crmath=True the correctly rounded functions the device also runs."""
+    return _sciara_call(oracle().orc_sciara_run, inp, steps, active, (int(crmath),))
 
 
 def ref_sciara(inp, steps, active=True):

@@ -82,9 +82,12 @@ BIN_SC = os.path.join(native.REF_DIR, "dropin_sciara")
 def test_reference_sciara_program_on_the_engine(tmp_path):
     """The reference SCIARA-fv2 program (main.cpp + its sources, one added bind
     call) on its own 2006 data: 1000 steps, full grid, saved with "%.9f".  The
-    GPU's files equal, as text, the oracle run with the same portable
-    elementary functions, and lie within 1e-9 relative of the glibc run (which
-    equals the reference program on the CPU)."""
+    GPU's files equal, as text, the oracle run with the same correctly
+    rounded elementary functions, and lie within 1e-11 of each field's scale
+    (plus the 5e-10 of the text format) of the glibc run, which equals the
+    reference program on the CPU: of the 780 000 saved values, 6 differ from
+    the reference program's in the ninth decimal (measured 2.2e-12 in h and
+    1.0e-9 K in T at most)."""
     import numpy as np
     import sciara2006
     print("case dropin sciara 2006 data 378x517 1000 steps", flush=True)
@@ -92,15 +95,15 @@ def test_reference_sciara_program_on_the_engine(tmp_path):
     r = subprocess.run([BIN_SC], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     inp = sciara2006.load_inputs()
-    port = native.oracle_sciara(inp, 1000, False, portable=True)
-    libm = native.oracle_sciara(inp, 1000, False, portable=False)
+    port = native.oracle_sciara(inp, 1000, False, crmath=True)
+    libm = native.oracle_sciara(inp, 1000, False, crmath=False)
     for name, k, scale in (("Thickness", "h", 1.0), ("Temperature", "T", 1400.0), ("Morphology", "z", 3400.0),
                            ("SolidifiedLavaThickness", "Msl", 1.0)):
         path = tmp_path / "data" / "2006_SAVE" / f"{sciara2006.PREFIX}_{name}.stt"
         text = path.read_text().split()[12:]
         assert text == ["%.9f" % v for v in port[k].ravel()], name
         got = np.array(text, dtype=np.float64)
-        assert np.abs(got - libm[k].ravel()).max() <= 1e-9 * scale + 1e-9, name
+        assert np.abs(got - libm[k].ravel()).max() <= 1e-11 * scale + 5e-10, name
 
 
 BIN_MB = os.path.join(native.REF_DIR, "dropin_mbusu")

@@ -1,15 +1,16 @@
 """SCIARA-fv2 on the GPU through the OpenCAL C API (examples/sciara.c bound
 with calhipBindSciara2D).
 
-Pins: (1) bit-exact agreement with the oracle run in portable-math mode (the
-same elementary functions, host-built) on seeded cases, active cells and
-full grid, including two vents on one cell and runs ended by the stop
-condition; (2) the steering twin equals the model's host steering bit for
-bit; (3) the libm oracle (bit-exact to the reference build on the CPU) within
-1e-9 relative: the elementary functions differ from glibc by a few ulp;
-(4) at 1024 x 1024 (size-independent properties): lava volume balance to
-1e-12 relative while lava stays off the border, temperature within
-[PTsol, PTvent], solidified altitude = initial altitude + Msl."""
+Pins: (1) bit-exact agreement with the oracle run with the correctly rounded
+functions of crmath.h (the same source, host-built) on seeded cases, active
+cells and full grid, including two vents on one cell and runs ended by the
+stop condition, and at 1024 x 1024 x 3000 steps; (2) the steering twin equals
+the model's host steering bit for bit; (3) the libm oracle (bit-exact to the
+reference build on the CPU) at 1024 x 1024 x 3000 steps, within the size of
+the reference's own spread between glibc's code paths; (4) at 1024 x 1024
+(size-independent properties): lava volume balance to 1e-12 relative while
+lava stays off the border, temperature within [PTsol, PTvent], solidified
+altitude = initial altitude + Msl."""
 import numpy as np
 import pytest
 
@@ -42,13 +43,13 @@ def _same(dev, ora, label):
 
 @pytest.mark.parametrize("active", [True, False], ids=["active", "full"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_sciara_bit_exact_vs_portable_oracle(case, active):
+def test_sciara_bit_exact_vs_crmath_oracle(case, active):
     from opencal_amd.models import sciara_run
     label, rows, cols, seed, vents, spr, nr, steps = case
     print(f"case sciara {label} active={active}", flush=True)
     inp = _inputs(rows, cols, seed, vents, spr, nr)
     dev = sciara_run(inp, steps, active)
-    ora = native.oracle_sciara(inp, steps, active, portable=True)
+    ora = native.oracle_sciara(inp, steps, active, crmath=True)
     _same(dev, ora, label)
     if active:
         assert dev["active"] == ora["active"]
@@ -65,15 +66,37 @@ def test_sciara_twin_equals_host_steering():
     assert a["active"] == b["active"] and a["elapsed"] == b["elapsed"] == 90 * inp["params"][0]
 
 
+def _rel(a, b):
+    d = np.abs(a - b)
+    s = np.maximum(np.abs(a), np.abs(b))
+    return np.where(s > 0, d / np.where(s > 0, s, 1.0), 0.0).max(), d.max() / max(s.max(), 1e-300)
+
+
 def test_sciara_close_to_libm_oracle():
+    """1024 x 1024, 3000 steps with active cells (3 vents, 10 rates of 300
+    steps).  The device equals the crmath oracle bit for bit.  Against the
+    libm oracle (the reference build's arithmetic) it differs only through
+    glibc's own misroundings (about one call in 10^3 to 10^4), which this
+    flow amplifies: measured 3.5e-11 (h), 8.5e-13 (T), 3.7e-16 (z) and 3.6e-11
+    (Msl) per cell, 8.6e-13 at most over each field's largest value.  The
+    reference build itself, run with glibc's SSE2 code path instead of its
+    FMA path on the same input, differs from its FMA run by 3.5e-11 (h),
+    8.5e-13 (T), 3.7e-16 (z) and 3.8e-9 (Msl) per cell, so a per-cell bound
+    of 1e-12 is not a property of the reference either.  Tolerances: 1e-9
+    per cell, 1e-11 over the field's largest value."""
     from opencal_amd.models import sciara_run
-    print("case sciara vs libm oracle", flush=True)
-    inp = _inputs(96, 96, 13, [[30, 48], [36, 40]], 50, 4)
-    dev = sciara_run(inp, 1000, True)
-    ref = native.oracle_sciara(inp, 1000, True, portable=False)
-    assert dev["steps"] == ref["steps"]
-    for k, scale in (("h", 1.0), ("T", 1400.0), ("z", 3000.0), ("Msl", 1.0)):
-        assert np.abs(dev[k] - ref[k]).max() <= 1e-9 * scale, k
+    print("case sciara 1024x1024 3000 steps vs crmath and libm oracles", flush=True)
+    inp = synth.sciara_inputs(1024, 1024, 17, steps_per_rate=300, n_rates=10)
+    dev = sciara_run(inp, 3000, True)
+    ora = native.oracle_sciara(inp, 3000, True, crmath=True)
+    _same(dev, ora, "1024 crmath")
+    assert dev["active"] == ora["active"]
+    ref = native.oracle_sciara(inp, 3000, True, crmath=False)
+    assert dev["steps"] == ref["steps"] == 3000 and dev["active"] == ref["active"]
+    for k in ("h", "T", "z", "Msl"):
+        cell, norm = _rel(dev[k], ref[k])
+        print(f"  {k}: per cell {cell:.2e}, over the field's largest value {norm:.2e}", flush=True)
+        assert cell <= 1e-9 and norm <= 1e-11, k
 
 
 def test_sciara_bit_exact_256_active():
@@ -81,7 +104,7 @@ def test_sciara_bit_exact_256_active():
     print("case sciara 256x256 1200 steps", flush=True)
     inp = synth.sciara_inputs(256, 256, 21, steps_per_rate=400, n_rates=3)
     dev = sciara_run(inp, 100000, True)
-    ora = native.oracle_sciara(inp, 100000, True, portable=True)
+    ora = native.oracle_sciara(inp, 100000, True, crmath=True)
     _same(dev, ora, "256")
     assert dev["active"] == ora["active"]
 
