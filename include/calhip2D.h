@@ -217,6 +217,21 @@ int calhipSciddicaTAdjointDev(int rows, int cols, const double* d_z, const doubl
                               double* d_lam, double* d_lam_z, double* d_r, double* d_eps,
                               double* timing_ms);
 
+/* As calhipSciddicaTForwardDev, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in OpenCAL.  It calls no hipMalloc,
+ * hipFree, stream or event creation: its one working grid and the sweep's
+ * spare row live in a fixed device array of the library (32 MiB + 1 KiB per
+ * GPU, held until the process ends), so grids are limited to 4194304 cells
+ * (e.g. 2048 x 2048; larger grids return -1 with error 51).  The sweeps run
+ * on the current device's null stream; the call returns when they are done.
+ * d_h_out may equal d_h0; d_ckpt may be NULL; no timing is reported.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (from
+ * several host threads) race. */
+int calhipSciddicaTForwardNoAllocDev(int rows, int cols, const double* d_z, const double* d_h0,
+                                     double p_r, double p_eps, int steps, int ck,
+                                     double* d_ckpt, double* d_h_out);
+
 /* As calhipSciddicaTAdjointDev, with dL/dz accumulated only for the cells of
  * the box [box_r0, box_r0 + box_rows) x [box_c0, box_c0 + box_cols) (the
  * source patch, whose altitude the init lowered by the initial thickness). */

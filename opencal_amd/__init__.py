@@ -80,6 +80,8 @@ def _declare(L):
                                           C.POINTER(dp), dp]
     L.calhipSciddicaTForwardDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                             vp, vp, dp]
+    L.calhipSciddicaTForwardNoAllocDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int,
+                                                   C.c_int, vp, vp]
     L.calhipSciddicaTAdjointDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                             vp, vp, dp, dp, dp]
     L.calhipSciddicaTAdjointBoxDev.argtypes = [C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,

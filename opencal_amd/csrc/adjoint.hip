@@ -30,6 +30,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -198,6 +199,38 @@ WorkCache& work()
 }
 enum { W_A, W_B, W_TRASH, W_STASH, W_PART, W_BX, W_INFO };
 
+// Scratch of the allocation-free forward (calhipSciddicaTForwardNoAllocDev):
+// one grid of the h ping-pong, at most NOALLOC_MAX_CELLS cells, then the
+// sweep's trash row.  Every GPU the library runs on holds its own copy until
+// the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device.
+constexpr size_t NOALLOC_MAX_CELLS = (size_t)1 << 22;   // 4 Mi cells: 32 MiB
+constexpr size_t NOALLOC_TRASH = 128;                   // doubles
+__device__ __align__(256) double scratch[NOALLOC_MAX_CELLS + NOALLOC_TRASH];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+double* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's scratch address, looked up on the first call on that
+// device (with the device current); nullptr with the error set on failure.
+double* noalloc_scratch()
+{
+	int dev = 0;
+	if (CALHIP_CHECK(hipGetDevice(&dev))) return nullptr;
+	if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+		set_error(52, "SciddicaT forward (no allocation): device index beyond 63");
+		return nullptr;
+	}
+	std::lock_guard<std::mutex> lk(noalloc_mu);
+	if (!noalloc_ptr[dev]) {
+		double* p = nullptr;
+		if (CALHIP_CHECK(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)))) return nullptr;
+		noalloc_ptr[dev] = p;
+	}
+	return noalloc_ptr[dev];
+}
+
 bool check_args(int rows, int cols, int steps, int ck)
 {
 	if (rows < 3 || cols < 3 || steps < 0 || ck <= 0) {
@@ -257,6 +290,49 @@ extern "C" int calhipSciddicaTForwardDev(int rows, int cols, const double* d_z, 
 	(void)hipStreamSynchronize(st);
 	(void)hipStreamDestroy(st);
 	return rc;
+}
+
+// The forward above without a device allocation, stream or event: the h
+// ping-pong alternates between d_h_out and the scratch grid so that the last
+// step lands in d_h_out, the sweeps run in plain tile order (every order
+// gives the same bits) with the shape already tuned for the grid, on the
+// current device's null stream.  Returns after the work has finished.
+extern "C" int calhipSciddicaTForwardNoAllocDev(int rows, int cols, const double* d_z, const double* d_h0,
+                                                double p_r, double p_eps, int steps, int ck, double* d_ckpt,
+                                                double* d_h_out)
+{
+	if (!check_args(rows, cols, steps, ck)) return -1;
+	if (!d_z || !d_h0 || !d_h_out) {
+		set_error(50, "SciddicaT forward (no allocation): null grid pointer");
+		return -1;
+	}
+	const size_t n = (size_t)rows * cols, bytes = n * sizeof(double);
+	if (n > NOALLOC_MAX_CELLS) {
+		set_error(51, "SciddicaT forward (no allocation): grid of " + std::to_string(n) +
+		                  " cells exceeds the scratch of 4194304 cells");
+		return -1;
+	}
+	double* base = noalloc_scratch();
+	if (!base) return -1;
+	double* const grid = base;
+	double* const trash = base + NOALLOC_MAX_CELLS;
+	const hipStream_t st = nullptr;
+	const SweepShape shape = sweep_shape_cached(rows, cols);
+	// h after t steps lives in d_h_out when steps - t is even
+	auto buf = [&](int t) -> double* { return ((steps - t) & 1) ? grid : d_h_out; };
+	if (buf(0) != d_h0 && CALHIP_CHECK(hipMemcpyAsync(buf(0), d_h0, bytes, hipMemcpyDeviceToDevice, st))) return -1;
+	for (int t = 0; t < steps; ++t) {
+		if (t % ck == 0 && d_ckpt &&
+		    CALHIP_CHECK(hipMemcpyAsync(d_ckpt + (size_t)(t / ck) * n, buf(t), bytes, hipMemcpyDeviceToDevice, st)))
+			return -1;
+		launch_sweep(st, rows, cols, d_z, buf(t), buf(t + 1), trash, p_r, p_eps, shape, nullptr);
+	}
+	if (steps % ck == 0 && d_ckpt &&
+	    CALHIP_CHECK(hipMemcpyAsync(d_ckpt + (size_t)(steps / ck) * n, d_h_out, bytes, hipMemcpyDeviceToDevice, st)))
+		return -1;
+	if (CALHIP_CHECK(hipGetLastError())) return -1;
+	if (CALHIP_CHECK(hipStreamSynchronize(st))) return -1;
+	return 0;
 }
 
 // d_lam: dL/dh_T in, dL/dh_0 out.  d_lam_z: dL/dz accumulated (+=) for every

@@ -650,6 +650,9 @@ void launch_sweep(hipStream_t st, int rows, int cols, const double* z, const dou
 // then cached per grid shape).
 SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch,
                        double* trash, double r, double eps);
+// The shape cached for this grid by an earlier sweep_shape call, else 16
+// rows at prefetch depth 1; times nothing and allocates nothing.
+SweepShape sweep_shape_cached(int rows, int cols);
 // Two steps per sweep (sciddicat_2step.hpp, flows reset between the steps):
 // the grids it supports (even cols >= 8, rows >= 8), one launch z, h -> hn,
 // and its tile shape (CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X = small-tile rows,

@@ -298,6 +298,13 @@ SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, cons
 	return best;
 }
 
+SweepShape sweep_shape_cached(int rows, int cols)
+{
+	std::lock_guard<std::mutex> lk(tune_mu);
+	const auto it = tuned.find(std::make_pair(rows, cols));
+	return it != tuned.end() ? it->second : SweepShape{16, 1};
+}
+
 Sweep2Shape sweep2_shape(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch,
                          double* trash, double r, double eps)
 {
