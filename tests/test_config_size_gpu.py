@@ -1,0 +1,108 @@
+"""Parity at the benchmark configurations' own sizes, through the C ABI.
+
+Pins: (1) SciddicaT 4096 x 4096 full grid, 100 steps of calRun2D's loop
+(calhipRunSteps2D, two steps per sweep with the steering twin), bit-exact
+against the oracle; (2) SciddicaT active cells 1024 x 1024 x 4000 steps: the
+grid and the final active-cell count bit-exact against the oracle; (3) the
+batched ensemble of 256 uneven instances (128..1024 per side), 100 steps:
+every instance bit-exact against the oracle; (4) the adjoint at 1024 x 1024 x
+2000 steps with checkpoints every 45: central finite differences (h = 1e-6)
+for p_r, p_eps and three source-patch cells, at points where FD with h and
+h/4 agree (away from elimination switching points), tolerance 1e-4.  The
+seeds are those bench.py measures."""
+import numpy as np
+import pytest
+
+import native
+from opencal_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_sciddicat_4096_two_step_loop_vs_oracle():
+    from opencal_amd.models import SciddicaT
+    print("case config 4096x4096 full grid, 100 steps of calhipRunSteps2D", flush=True)
+    z, h, _ = synth.sciddicat_inputs(4096, 4096, 2024)
+    m = SciddicaT(z, h, 0.5, 0.001, 100, False)
+    try:
+        m.init()
+        m.run_steps(100)
+        gz, gh, _ = m.read()
+        st = m.stats()
+    finally:
+        m.close()
+    assert st["steps"] == 100 and st["launches"] < 100   # two steps per sweep
+    oz, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 100, False)
+    assert np.array_equal(gz, oz)
+    assert np.array_equal(gh, oh), f"max |dh| = {np.abs(gh - oh).max()}"
+    assert (gh != h).sum() > 1000
+
+
+def test_sciddicat_active_1024_x4000_vs_oracle():
+    from opencal_amd.models import sciddicat_run
+    print("case config 1024x1024 active cells, 4000 steps", flush=True)
+    z, h, _ = synth.sciddicat_inputs(1024, 1024, 7)
+    gz, gh, gcount = sciddicat_run(z, h, 0.5, 0.001, 4000, active=True)
+    oz, oh, counts = native.oracle_sciddicat(z, h, 0.5, 0.001, 4000, True, counts=True)
+    assert np.array_equal(gz, oz)
+    assert np.array_equal(gh, oh), f"max |dh| = {np.abs(gh - oh).max()}"
+    assert gcount == counts[-1] and gcount > 1000
+
+
+def test_batch_256_instances_vs_oracle():
+    from opencal_amd.models import sciddicat_batch
+    print("case config batch 256 instances 128..1024, 100 steps", flush=True)
+    specs = synth.batch_instances(256, 4)
+    insts = []
+    for rows, cols, pr, pe, s in specs:
+        z, h, _ = synth.sciddicat_inputs(rows, cols, s)
+        insts.append((z, h, pr, pe))
+    outs, ms = sciddicat_batch(insts, 100)
+    assert ms > 0
+    native.set_threads(16)
+    bad = []
+    for k, ((z, h, pr, pe), got) in enumerate(zip(insts, outs)):
+        _, want = native.oracle_sciddicat(z, h, pr, pe, 100, False)
+        if not np.array_equal(got, want):
+            bad.append((k, z.shape))
+    assert not bad, f"instances differing from the oracle: {bad[:10]}"
+
+
+def test_adjoint_1024_x2000_finite_differences():
+    import torch
+    from opencal_amd.adjoint import forward_only, sciddicat_run
+    print("case config adjoint 1024x1024 x2000 ck45, FD on p_r, p_eps and 3 patch cells", flush=True)
+    steps, ck = 2000, 45
+    z_np, h_np, (r0, c0, p) = synth.sciddicat_inputs(1024, 1024, 5)
+    z_dem = torch.tensor(z_np, dtype=torch.float64, device="cuda")
+    patch0 = torch.tensor(h_np[r0:r0 + p, c0:c0 + p].copy(), dtype=torch.float64, device="cuda")
+    target = forward_only(z_dem, patch0, (r0, c0), 0.6, 0.003, steps)
+
+    def loss(prv, pev, q):
+        with torch.no_grad():
+            return ((forward_only(z_dem, q, (r0, c0), prv, pev, steps) - target) ** 2).sum().item()
+
+    pr = torch.tensor(0.5, dtype=torch.float64, requires_grad=True)
+    pe = torch.tensor(0.001, dtype=torch.float64, requires_grad=True)
+    patch = patch0.clone().requires_grad_(True)
+    hT = sciddicat_run(pr, pe, patch, z_dem, (r0, c0), steps, ck)
+    ((hT - target) ** 2).sum().backward()
+
+    checks = [("p_r", lambda d: loss(0.5 + d, 0.001, patch0), pr.grad.item()),
+              ("p_eps", lambda d: loss(0.5, 0.001 + d, patch0), pe.grad.item())]
+    for (i, j) in [(0, 0), (p // 2, p // 3), (p - 1, p - 2)]:
+        def along(d, i=i, j=j):
+            q = patch0.clone()
+            q[i, j] += d
+            return loss(0.5, 0.001, q)
+        checks.append((f"patch[{i},{j}]", along, patch.grad[i, j].item()))
+    checked = []
+    for name, fn, g in checks:
+        f1 = (fn(1e-6) - fn(-1e-6)) / 2e-6
+        f2 = (fn(2.5e-7) - fn(-2.5e-7)) / 5e-7
+        smooth = abs(f1 - f2) <= 1e-4 * max(1.0, abs(f1))
+        print(f"  {name}: adjoint {g:.10e} fd {f1:.10e} fd(h/4) {f2:.10e} smooth={smooth}", flush=True)
+        if smooth:
+            assert abs(g - f1) <= 1e-4 * max(1.0, abs(f1)), name
+            checked.append(name)
+    assert "p_r" in checked and "p_eps" in checked and len(checked) >= 4, checked
