@@ -122,8 +122,10 @@ void calUpdateSubstate2Di(struct CALModel2D* ca2D, struct CALSubstate2Di* Q);
 void calUpdateSubstate2Dr(struct CALModel2D* ca2D, struct CALSubstate2Dr* Q);
 
 /* cal2D.c:717 — one elementary process over the grid / active cells.  Runs
- * the process's device twin when its program supports single-process sweeps;
- * otherwise reports an error through calhipGetLastError(). */
+ * the process's device twin (generic, SciddicaT and Life programs: reads the
+ * current layers, writes the next ones; the caller's update brings them in);
+ * on the SCIARA program, whose kernels merge processes, it reports error 3
+ * through calhipGetLastError(). */
 void calApplyElementaryProcess2D(struct CALModel2D* ca2D, CALCallbackFunc2D elementary_process);
 
 /* cal2D.c:738 — every elementary process in order, each followed by
@@ -132,7 +134,8 @@ void calApplyElementaryProcess2D(struct CALModel2D* ca2D, CALCallbackFunc2D elem
 void calGlobalTransitionFunction2D(struct CALModel2D* ca2D);
 
 /* cal2D.c:758 — rebuild the active set (active mode), then next -> current
- * for every double-layer substate.  On the device this is a buffer flip. */
+ * for every double-layer substate.  On the device: a buffer flip of each
+ * written layer (full grid), or its copy at the active cells (active mode). */
 void calUpdate2D(struct CALModel2D* ca2D);
 
 /* cal2D.c:780-930 — per-cell accessors (host side; they bring the model's

@@ -39,6 +39,8 @@ CALreal calReductionComputeMin2Dr(struct CALModel2D* model, struct CALSubstate2D
 CALbyte calReductionComputeSum2Db(struct CALModel2D* model, struct CALSubstate2Db* substate);
 CALint calReductionComputeSum2Di(struct CALModel2D* model, struct CALSubstate2Di* substate);
 CALreal calReductionComputeSum2Dr(struct CALModel2D* model, struct CALSubstate2Dr* substate);
+/* As in the reference (cal2DReduction.c:563-571): the byte variant is the
+ * product of the cells, the int and real variants return their SUM. */
 CALbyte calReductionComputeProd2Db(struct CALModel2D* model, struct CALSubstate2Db* substate);
 CALint calReductionComputeProd2Di(struct CALModel2D* model, struct CALSubstate2Di* substate);
 CALreal calReductionComputeProd2Dr(struct CALModel2D* model, struct CALSubstate2Dr* substate);

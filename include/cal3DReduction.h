@@ -39,6 +39,8 @@ CALreal calReductionComputeMin3Dr(struct CALModel3D* model, struct CALSubstate3D
 CALbyte calReductionComputeSum3Db(struct CALModel3D* model, struct CALSubstate3Db* substate);
 CALint calReductionComputeSum3Di(struct CALModel3D* model, struct CALSubstate3Di* substate);
 CALreal calReductionComputeSum3Dr(struct CALModel3D* model, struct CALSubstate3Dr* substate);
+/* As in the reference (cal3DReduction.c:563-571): the byte variant is the
+ * product of the cells, the int and real variants return their SUM. */
 CALbyte calReductionComputeProd3Db(struct CALModel3D* model, struct CALSubstate3Db* substate);
 CALint calReductionComputeProd3Di(struct CALModel3D* model, struct CALSubstate3Di* substate);
 CALreal calReductionComputeProd3Dr(struct CALModel3D* model, struct CALSubstate3Dr* substate);

@@ -97,6 +97,7 @@ def _declare_examples(E):
     E.sciddicat_steps.argtypes = [vp, C.c_int]
     E.sciddicat_run_steps.argtypes = [vp, C.c_int]
     E.sciddicat_read.argtypes = [vp, dp, dp, ip]
+    E.sciddicat_set_explicit.argtypes = [vp, C.c_int]
     E.sciddicat_read_flow.argtypes = [vp, C.c_int, dp]
     E.sciddicat_model.restype = vp
     E.sciddicat_model.argtypes = [vp]
@@ -106,6 +107,7 @@ def _declare_examples(E):
     E.life_run.argtypes = [vp]
     E.life_steps.argtypes = [vp, C.c_int]
     E.life_read.argtypes = [vp, bp]
+    E.life_set_explicit.argtypes = [vp, C.c_int]
     E.life_model.restype = vp
     E.life_model.argtypes = [vp]
     E.life_destroy.argtypes = [vp]
@@ -147,6 +149,8 @@ def _declare_examples(E):
     E.gred_create.argtypes = [C.c_int, C.c_int, sbp, ip, dp]
     E.gred_reduce.restype = C.c_double
     E.gred_reduce.argtypes = [vp, C.c_int, C.c_int]
+    E.gred_compute_prod.restype = C.c_double
+    E.gred_compute_prod.argtypes = [vp, C.c_int]
     E.gred_step.argtypes = [vp]
     E.gred_on_device.argtypes = [vp]
     E.gred_destroy.argtypes = [vp]
@@ -160,6 +164,8 @@ def _declare_examples(E):
     E.gheat_run.argtypes = [vp, dp, bp, ip]
     E.gheat_reduce.restype = C.c_double
     E.gheat_reduce.argtypes = [vp, C.c_int, C.c_int]
+    E.gheat_compute_prod.restype = C.c_double
+    E.gheat_compute_prod.argtypes = [vp, C.c_int]
     E.gheat_load.argtypes = [vp, dp, bp, ip]
     E.gheat_on_device.argtypes = [vp]
     E.gheat_step.argtypes = [vp]
@@ -167,6 +173,7 @@ def _declare_examples(E):
     E.ginf_create.restype = vp
     E.ginf_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_int, C.c_int]
     E.ginf_steps.argtypes = [vp, C.c_int]
+    E.ginf_apply_steps.argtypes = [vp, C.c_int]
     E.ginf_read.argtypes = [vp, bp, bp, ip, ip, ip, C.c_int]
     E.ginf_destroy.argtypes = [vp]
     E.ggift3_create.restype = vp

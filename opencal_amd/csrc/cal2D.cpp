@@ -411,7 +411,8 @@ static void update_one(CALModel2D* m, void* q)
 	ModelX* x = X(m);
 	SubX* s = S(q);
 	if (x->phase == Phase::Device && !s->host_only) {
-		if (s->nxt_dirty) flip(s);
+		device_changed(x);
+		device_update_sub(x, s);
 		return;
 	}
 	host_update_sub(m, s);

@@ -317,7 +317,7 @@ static void update_one3(CALModel3D* m, void* q)
 	SubX* s = S(q);
 	if (c->phase == Phase::Device) {
 		device_changed(c);
-		if (s->nxt_dirty) flip(s);
+		device_update_sub(c, s);
 		return;
 	}
 	host_update_sub3(m, s);

@@ -205,6 +205,7 @@ inline int ensure_host_for(ModelX* m, void* q) { return S(q)->host_only ? 0 : en
 int ensure_device(ModelX* m);  // HOST -> DEVICE (no-op in DEVICE)
 int ensure_device_buffers(ModelX* m, SubX* s);
 int materialise_next(ModelX* m, SubX* s);  // stale next := current
+int device_update_sub(ModelX* m, SubX* s);  // calUpdateSubstate of one written layer
 int device_fill(ModelX* m, SubX* s, double v, bool both);
 int device_update(ModelX* m);
 int device_rebuild_active(ModelX* m);

@@ -197,7 +197,8 @@ struct LifeProgram : Program {
 	SubX* q = nullptr;
 	const char* name() const override { return "Life"; }
 
-	int launch_step(ModelX* m)
+	// one sweep current -> next (no flip)
+	int launch_sweep_only(ModelX* m)
 	{
 		size_t n = m->cells();
 		int grid = (int)std::min<size_t>((n + 255) / 256, 16384);
@@ -211,8 +212,29 @@ struct LifeProgram : Program {
 			                   (int32_t*)q->d_nxt, m->pub.rows, m->pub.columns);
 		timer_end(m, ev);
 		m->stats.launches++;
-		if (CALHIP_CHECK(hipGetLastError())) return -1;
+		return CALHIP_CHECK(hipGetLastError());
+	}
+
+	int launch_step(ModelX* m)
+	{
+		if (launch_sweep_only(m)) return -1;
 		flip(q);
+		return 0;
+	}
+
+	// calApplyElementaryProcess2D (cal2D.c:717-734) with the Life transition:
+	// the sweep writes every cell of the next layer; the caller's
+	// calUpdate2D (or calUpdateSubstate2D*) flips it in.
+	int apply(ModelX* m, const void* ep) override
+	{
+		if (m->prog_eps.empty() || (const void*)m->prog_eps[0] != ep) {
+			set_error(33, "Life device program: calApplyElementaryProcess2D with a process that is not the bound "
+			              "transition");
+			return -1;
+		}
+		if (launch_sweep_only(m)) return -1;
+		q->nxt_stale = false;
+		q->nxt_dirty = true;
 		return 0;
 	}
 

@@ -282,6 +282,16 @@ inline ModelX* owner_of(void* q) { return S(q)->owner; }
 
 using namespace calhip;
 
+// calReductionComputeProd*: the reference's byte variants reduce with
+// REDUCTION_PROD but its int and real variants with REDUCTION_SUM
+// (OpenCAL-OMP/source/cal2DReduction.c:563-571, cal3DReduction.c:563-571); a
+// drop-in model gets the same values here.
+template <typename T>
+constexpr REDUCTION_OPERATION compute_prod_op()
+{
+	return sizeof(T) == 1 ? REDUCTION_PROD : REDUCTION_SUM;
+}
+
 #define RED_FUNCS(D, SUF, T)                                                                                     \
 	extern "C" T calReductionOperation##D##SUF(struct CALModel##D* model, struct CALSubstate##D##SUF* substate,   \
 	                                            enum REDUCTION_OPERATION operation)                              \
@@ -309,7 +319,7 @@ using namespace calhip;
 	}                                                                                                            \
 	extern "C" T calReductionComputeProd##D##SUF(struct CALModel##D* m, struct CALSubstate##D##SUF* q)           \
 	{                                                                                                            \
-		return calReductionOperation##D##SUF(m, q, REDUCTION_PROD);                                              \
+		return calReductionOperation##D##SUF(m, q, compute_prod_op<T>());                                        \
 	}                                                                                                            \
 	extern "C" T calReductionComputeLogicalAnd##D##SUF(struct CALModel##D* m, struct CALSubstate##D##SUF* q)     \
 	{                                                                                                            \

@@ -274,13 +274,32 @@ int device_rebuild_active(ModelX* m)
 
 int Program::device_update_hook(ModelX* m) { return device_update(m); }
 
-// calUpdate2D on the device: rebuild, then flip every written next layer.
+// calUpdateSubstate2D* of one written next layer on the device
+// (cal2D.c:694-713): in active mode next -> current at the cells of the
+// active list (the whole grid while it is empty), so that a cell written by
+// a process and then removed from the set keeps its current value as in the
+// reference; otherwise a flip of the two buffers.
+int device_update_sub(ModelX* m, SubX* s)
+{
+	if (s->host_only || s->single || !s->nxt_dirty) return 0;
+	if (!m->active()) {
+		flip(s);
+		return 0;
+	}
+	if (m->prog && m->prog->sync_active(m)) return -1;
+	if (k::copy_active(m, s->d_cur, s->d_nxt, s->elem)) return -1;
+	s->nxt_dirty = false;
+	s->nxt_stale = false;   // the next layer keeps its own values outside the set
+	return 0;
+}
+
+// calUpdate2D on the device: rebuild, then update every written next layer.
 int device_update(ModelX* m)
 {
 	device_changed(m);
 	if (device_rebuild_active(m)) return -1;
 	for (SubX* s : m->subs)
-		if (!s->host_only && !s->single && s->nxt_dirty) flip(s);
+		if (device_update_sub(m, s)) return -1;
 	if (m->prog) m->prog->transient_update(m);
 	return 0;
 }

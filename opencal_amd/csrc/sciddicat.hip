@@ -137,9 +137,10 @@ __global__ void flows_active(const double* __restrict__ z, const double* __restr
 	}
 }
 
-// width over the active list, in place (the process reads only its own h),
-// followed by the removal of cells with h <= eps.
-__global__ void width_active(double* __restrict__ h, FlowPtrs f, uint8_t* __restrict__ flags,
+// width over the active list from hin into hout (the same array for the
+// in-place step: the process reads only its own h), optionally followed by
+// the removal of cells with h <= eps.
+__global__ void width_active(const double* hin, double* hout, FlowPtrs f, uint8_t* __restrict__ flags,
                              const int* __restrict__ cells, const int* __restrict__ count, int rows, int cols,
                              double eps, int remove)
 {
@@ -152,13 +153,26 @@ __global__ void width_active(double* __restrict__ h, FlowPtrs f, uint8_t* __rest
 		size_t cw = (size_t)i * cols + dev::wrap1(j - 1, cols);
 		size_t ce = (size_t)i * cols + dev::wrap1(j + 1, cols);
 		size_t cs = (size_t)dev::wrap1(i + 1, rows) * cols + j;
-		double o = h[c];
+		double o = hin[c];
 		o += f.f[3][cn] - f.f[0][c];
 		o += f.f[2][cw] - f.f[1][c];
 		o += f.f[1][ce] - f.f[2][c];
 		o += f.f[0][cs] - f.f[3][c];
-		h[c] = o;
+		hout[c] = o;
 		if (remove && o <= eps) flags[c] = 0;
+	}
+}
+
+// the removal process alone (active variant sciddicaT.c:94-100): cells of
+// the list (or of the grid when it is empty) with h <= eps leave the set
+__global__ void remove_active(const double* __restrict__ h, uint8_t* __restrict__ flags,
+                              const int* __restrict__ cells, const int* __restrict__ count, size_t n, double eps)
+{
+	const int cnt = count[0];
+	const size_t total = cnt > 0 ? (size_t)cnt : n;
+	for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (size_t)gridDim.x * blockDim.x) {
+		const size_t c = cnt > 0 ? (size_t)cells[k] : k;
+		if (h[c] <= eps) flags[c] = 0;
 	}
 }
 
@@ -633,8 +647,8 @@ struct SciddicaTProgram : Program {
 		                   eps, zero_dry);
 		m->flags_dirty = true;
 		if (device_rebuild_active(m)) return -1;
-		hipLaunchKernelGGL(width_active, dim3(grid), dim3(256), 0, m->stream, (double*)h->d_cur, cur_flows(),
-		                   m->d_flags, m->d_cells, m->d_count, rows, cols, eps, 1);
+		hipLaunchKernelGGL(width_active, dim3(grid), dim3(256), 0, m->stream, (const double*)h->d_cur,
+		                   (double*)h->d_cur, cur_flows(), m->d_flags, m->d_cells, m->d_count, rows, cols, eps, 1);
 		m->flags_dirty = true;
 		if (device_rebuild_active(m)) return -1;
 		timer_end(m, ev);
@@ -689,6 +703,84 @@ struct SciddicaTProgram : Program {
 			n -= 2;
 		}
 		return Program::run_steps(m, n, true);
+	}
+
+	// calApplyElementaryProcess2D (cal2D.c:717-734) with one of the bound
+	// processes: that sweep alone, reading the current layers and writing the
+	// next ones, over the grid or the active list (the whole grid while it is
+	// empty); the caller's calUpdate2D then updates the written layers.  The
+	// flows become ordinary substates first.
+	int apply(ModelX* m, const void* ep) override
+	{
+		int which = -1;
+		for (size_t k = 0; k < m->prog_eps.size(); ++k)
+			if ((const void*)m->prog_eps[k] == ep) which = (int)k;
+		if (which < 0) {
+			set_error(12, "SciddicaT device program: calApplyElementaryProcess2D with a process that is not one "
+			              "of the bound flows, width or remove");
+			return -1;
+		}
+		for (int q = 0; q < 4; ++q)
+			if (f[q]->tstate != Transient::None) {
+				if (materialise(m, f[q])) return -1;
+				m->stats.materialisations++;
+			}
+		h_prev_valid = false;
+		const double r = *p_r, eps = *p_eps;
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		const size_t n = m->cells();
+		const int grid = stride_grid(n, 256);
+		if (active) {
+			if (sync_active(m)) return -1;   // the list of the last update
+			ctr_valid = false;               // the tile lists are rebuilt from the flags
+		}
+		FlowPtrs fn;
+		for (int q = 0; q < 4; ++q) fn.f[q] = (double*)f[q]->d_nxt;
+		hipEvent_t ev;
+		timer_begin(m, &ev);
+		if (which == 0) {
+			// flows: the next layer at wet cells; elsewhere it keeps its value
+			// (equal to current unless an earlier process wrote it)
+			bool stale = true;
+			for (int q = 0; q < 4; ++q) stale = stale && f[q]->nxt_stale;
+			if (active || !stale)
+				for (int q = 0; q < 4; ++q)
+					if (materialise_next(m, f[q])) return -1;
+			if (active) {
+				hipLaunchKernelGGL(flows_active, dim3(grid), dim3(256), 0, m->stream, (const double*)z->d_cur,
+				                   (const double*)h->d_cur, fn, m->d_flags, m->d_cells, m->d_count, rows, cols, r,
+				                   eps, 0);
+				m->flags_dirty = true;   // receivers joined the set
+			} else {
+				// with stale next layers the kernel copies current into next at
+				// dry cells; otherwise (fcur = fnxt) it leaves them
+				hipLaunchKernelGGL(flows_full, dim3(grid), dim3(256), 0, m->stream, (const double*)z->d_cur,
+				                   (const double*)h->d_cur, stale ? cur_flows() : fn, fn, rows, cols, r, eps);
+			}
+			for (int q = 0; q < 4; ++q) {
+				f[q]->nxt_stale = false;
+				f[q]->nxt_dirty = true;
+			}
+		} else if (which == 1) {
+			if (active) {
+				if (materialise_next(m, h)) return -1;
+				hipLaunchKernelGGL(width_active, dim3(grid), dim3(256), 0, m->stream, (const double*)h->d_cur,
+				                   (double*)h->d_nxt, cur_flows(), m->d_flags, m->d_cells, m->d_count, rows, cols,
+				                   eps, 0);
+			} else {
+				hipLaunchKernelGGL(width_full, dim3(grid), dim3(256), 0, m->stream, (const double*)h->d_cur,
+				                   cur_flows(), (double*)h->d_nxt, rows, cols);
+			}
+			h->nxt_stale = false;
+			h->nxt_dirty = true;
+		} else {
+			hipLaunchKernelGGL(remove_active, dim3(grid), dim3(256), 0, m->stream, (const double*)h->d_cur,
+			                   m->d_flags, m->d_cells, m->d_count, n, eps);
+			m->flags_dirty = true;
+		}
+		timer_end(m, ev);
+		m->stats.launches++;
+		return CALHIP_CHECK(hipGetLastError());
 	}
 
 	int transient_init(ModelX* m, SubX* s, double v, bool both) override

@@ -422,6 +422,15 @@ double gred_reduce(void* handle, int which, int op)
 	return calReductionOperation2Dr(s->model, gredS.r, (enum REDUCTION_OPERATION)op);
 }
 
+/* calReductionComputeProd2D{b,i,r} (which = 0, 1, 2) as a double */
+double gred_compute_prod(void* handle, int which)
+{
+	gred_t* s = (gred_t*)handle;
+	if (which == 0) return (double)calReductionComputeProd2Db(s->model, gredS.b);
+	if (which == 1) return (double)calReductionComputeProd2Di(s->model, gredS.i);
+	return calReductionComputeProd2Dr(s->model, gredS.r);
+}
+
 /* a transition that keeps every value: the model moves to the GPU */
 int gred_step(void* handle)
 {

@@ -217,6 +217,15 @@ double gheat_reduce(void* handle, int which, int op)
 	return (double)calReductionOperation3Di(s->model, gheatS.count, (enum REDUCTION_OPERATION)op);
 }
 
+/* calReductionComputeProd3D of T (0), hot (1) or count (2), as a double */
+double gheat_compute_prod(void* handle, int which)
+{
+	gheat_t* s = (gheat_t*)handle;
+	if (which == 0) return calReductionComputeProd3Dr(s->model, gheatS.T);
+	if (which == 1) return (double)calReductionComputeProd3Db(s->model, gheatS.hot);
+	return (double)calReductionComputeProd3Di(s->model, gheatS.count);
+}
+
 /* overwrite the three substates from the host (the model returns to the host) */
 int gheat_load(void* handle, const double* T, const unsigned char* hot, const int* count)
 {
@@ -334,6 +343,23 @@ int ginf_steps(void* handle, int n)
 	ginf_t* s = (ginf_t*)handle;
 	int k;
 	for (k = 0; k < n; k++) calGlobalTransitionFunction3D(s->model);
+	return calhipGetLastError();
+}
+
+/* the same steps with each process run through calApplyElementaryProcess3D
+ * and followed by calUpdate3D (the sequence of calGlobalTransitionFunction3D,
+ * cal3D.c:677-695, written out by the model): ginf_retire writes a cell's
+ * next layer and removes the cell, which the update then leaves out */
+int ginf_apply_steps(void* handle, int n)
+{
+	ginf_t* s = (ginf_t*)handle;
+	int k;
+	for (k = 0; k < n; k++) {
+		calApplyElementaryProcess3D(s->model, ginf_spread);
+		calUpdate3D(s->model);
+		calApplyElementaryProcess3D(s->model, ginf_retire);
+		calUpdate3D(s->model);
+	}
 	return calhipGetLastError();
 }
 

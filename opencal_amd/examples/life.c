@@ -44,6 +44,32 @@ void* life_create(int rows, int cols, const unsigned char* alive0, int steps)
 	return s;
 }
 
+/* A global transition of the model's own: the process applied alone, then
+ * calUpdateSubstate2Db (mode 1) or calUpdate2D (mode 2). */
+static void life_explicit_substate(struct CALModel2D* ca)
+{
+	calApplyElementaryProcess2D(ca, life_transition);
+	calUpdateSubstate2Db(ca, L->Q);
+}
+
+static void life_explicit_update(struct CALModel2D* ca)
+{
+	calApplyElementaryProcess2D(ca, life_transition);
+	calUpdate2D(ca);
+}
+
+int life_set_explicit(void* h, int mode)
+{
+	life_t* s = (life_t*)h;
+	if (mode == 1)
+		calRunAddGlobalTransitionFunc2D(s->run, life_explicit_substate);
+	else if (mode == 2)
+		calRunAddGlobalTransitionFunc2D(s->run, life_explicit_update);
+	else
+		return -1;
+	return 0;
+}
+
 int life_run(void* h)
 {
 	life_t* s = (life_t*)h;

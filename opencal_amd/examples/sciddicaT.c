@@ -119,6 +119,33 @@ static CALbyte sciddicaT_stop(struct CALModel2D* ca)
 	return S->run->step >= S->steps ? CAL_TRUE : CAL_FALSE;
 }
 
+/* A model's own global transition that runs the processes one at a time,
+ * as models with a custom transition do (e.g.
+ * OpenCAL/examples/cal_sciddicaS3hex-unsafe-explicit-glut/source/
+ * sciddicaS3Hex.c:218-229): each calApplyElementaryProcess2D followed by
+ * calUpdate2D, or (full grid) by calUpdateSubstate2Dr of the substates the
+ * process wrote. */
+static void sciddicaT_explicit_update2D(struct CALModel2D* ca)
+{
+	calApplyElementaryProcess2D(ca, sciddicaT_flows_computation);
+	calUpdate2D(ca);
+	calApplyElementaryProcess2D(ca, sciddicaT_width_update);
+	calUpdate2D(ca);
+	if (S->active) {
+		calApplyElementaryProcess2D(ca, sciddicaT_remove_inactive_cells);
+		calUpdate2D(ca);
+	}
+}
+
+static void sciddicaT_explicit_substates(struct CALModel2D* ca)
+{
+	CALint k;
+	calApplyElementaryProcess2D(ca, sciddicaT_flows_computation);
+	for (k = 0; k < NUMBER_OF_OUTFLOWS; k++) calUpdateSubstate2Dr(ca, S->Q.f[k]);
+	calApplyElementaryProcess2D(ca, sciddicaT_width_update);
+	calUpdateSubstate2Dr(ca, S->Q.h);
+}
+
 /* ---------------------------------------------------------------- entry points */
 
 /* Build the model exactly as the reference examples do, load z and h (the
@@ -159,6 +186,21 @@ void* sciddicat_create(int rows, int cols, const double* z, const double* h, dou
 	    calhipBindSciddicaTSteering2D(s->model, sciddicaT_steering))
 		return NULL;
 	return s;
+}
+
+/* Install a global transition of the model's own (mode 1: each process
+ * followed by calUpdate2D; mode 2, full grid: followed by calUpdateSubstate2Dr
+ * of what it wrote); the run loop then calls it with the implicit update. */
+int sciddicat_set_explicit(void* handle, int mode)
+{
+	sciddicat_t* s = (sciddicat_t*)handle;
+	if (mode == 1)
+		calRunAddGlobalTransitionFunc2D(s->run, sciddicaT_explicit_update2D);
+	else if (mode == 2 && !s->active)
+		calRunAddGlobalTransitionFunc2D(s->run, sciddicaT_explicit_substates);
+	else
+		return -1;
+	return 0;
 }
 
 /* calRun2D: init, all steps, finalize (state back on the host afterwards). */

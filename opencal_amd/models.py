@@ -54,6 +54,12 @@ class SciddicaT:
         with its steering twin advances two steps per sweep."""
         check(self.E.sciddicat_run_steps(self.handle, int(n)), "sciddicat_run_steps")
 
+    def set_explicit(self, mode):
+        """Run the processes one at a time from a global transition of the
+        model's own: mode 1 with calUpdate2D after each, mode 2 (full grid)
+        with calUpdateSubstate2Dr of the written substates."""
+        check(self.E.sciddicat_set_explicit(self.handle, int(mode)), "sciddicat_set_explicit")
+
     def sync(self):
         check(self.E.sciddicat_sync(self.handle), "sciddicat_sync")
 
@@ -123,6 +129,11 @@ class Life:
 
     def steps(self, n):
         check(self.E.life_steps(self.handle, int(n)), "life_steps")
+
+    def set_explicit(self, mode):
+        """The transition applied alone from the model's own global
+        transition, then calUpdateSubstate2Db (mode 1) or calUpdate2D (2)."""
+        check(self.E.life_set_explicit(self.handle, int(mode)), "life_set_explicit")
 
     def read(self):
         out = np.empty((self.rows, self.cols), np.uint8)

@@ -49,6 +49,7 @@ def _ref3():
     L.ginf_create.restype = vp
     L.ginf_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int, C.c_int]
     L.ginf_steps.argtypes = [vp, C.c_int]
+    L.ginf_apply_steps.argtypes = [vp, C.c_int]
     L.ginf_read.argtypes = [vp, _bp, _bp, _ip, _ip, _ip, C.c_int]
     L.ginf_destroy.argtypes = [vp]
     L.ggift3_create.restype = vp
@@ -267,10 +268,11 @@ def test_oracle_vs_reference_engine_3d():
     for nb in ("von_neumann", "moore"):
         for torus in (0, 1):
             shape, seeds = (5, 6, 7), [(0, 0, 0), (3, 2, 4)]
-            got = _run_inf(R, shape, nb, torus, seeds, 3, 7)
             exp = oracle_inf(shape, nb, bool(torus), seeds, 3, 7)
-            for a, b in zip(got, exp):
-                assert np.array_equal(a, b), (nb, torus)
+            for apply in (False, True):
+                got = _run_inf(R, shape, nb, torus, seeds, 3, 7, apply)
+                for a, b in zip(got, exp):
+                    assert np.array_equal(a, b), (nb, torus, apply)
     # unsafe operations (torus: on a flat box the reference's additions
     # leave its arrays at the border)
     for nb in ("von_neumann", "moore"):
@@ -317,13 +319,14 @@ def test_reduction_vectorised_equals_the_loop():
                 assert same, (t, op, a, b)
 
 
-def _run_inf(L, shape, nb, torus, seeds, span, steps):
+def _run_inf(L, shape, nb, torus, seeds, span, steps, apply=False):
     s, r, c = shape
     sd = np.array([[i, j, k] for (i, j, k) in seeds], np.int32).reshape(-1)
     hd = L.ginf_create(r, c, s, NB[nb], torus, sd.ctypes.data_as(_ip), len(seeds), span)
     assert hd, lib().calhipGetErrorString()
     try:
-        assert L.ginf_steps(hd, steps) == 0, lib().calhipGetErrorString()
+        run = L.ginf_apply_steps if apply else L.ginf_steps
+        assert run(hd, steps) == 0, lib().calhipGetErrorString()
         sc = np.zeros(shape, np.uint8)
         sn = np.zeros(shape, np.uint8)
         ac = np.zeros(shape, np.int32)
@@ -432,6 +435,27 @@ def test_active_cells_3d_gpu(nb, torus):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("nb", ["von_neumann", "moore"])
+def test_active_cells_3d_apply_then_update_gpu(nb):
+    """Each process through calApplyElementaryProcess3D followed by
+    calUpdate3D: the retiring process writes a cell's next layer and removes
+    the cell, so the update (at the cells of the rebuilt set, cal3D.c:677-715;
+    the 2D engine's update is the same code) leaves that cell's current layer
+    as it was, and both layers equal the oracle's (the reference engine's
+    own sequence, pinned against the reference build above)."""
+    print(f"case 3d active apply+update nb={nb}", flush=True)
+    E = examples()
+    shape, seeds = (8, 9, 10), [(0, 0, 0), (4, 5, 3), (8, 2, 7)]
+    lib().calhipClearError()
+    got = _run_inf(E, shape, nb, 1, seeds, 3, 9, apply=True)
+    exp = oracle_inf(shape, nb, True, seeds, 3, 9)
+    names = ["s current", "s next", "age current", "age next", "active list"]
+    for a, b, n in zip(got, exp, names):
+        assert np.array_equal(a, b), n
+    assert (got[0] != got[1]).any()   # retired cells: next 2, current kept
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nb", ["von_neumann", "moore"])
 @pytest.mark.parametrize("torus", [0, 1], ids=["flat", "toroidal"])
 def test_unsafe_operations_3d_gpu(nb, torus):
     print(f"case 3d unsafe nb={nb} torus={torus}", flush=True)
@@ -456,6 +480,44 @@ def test_unsafe_operations_3d_gpu(nb, torus):
         assert np.array_equal(q, eq + eq1)
     finally:
         E.ggift3_destroy(hd)
+
+
+# the operation calReductionComputeProd{2D,3D}{b,i,r} reduces with: the
+# reference's wrappers use PROD for bytes and SUM for ints and reals
+# (OpenCAL-OMP/source/cal2DReduction.c:563-571, cal3DReduction.c:563-571)
+PROD_WRAPPER_OP = {"b": O.OPS.index("PROD"), "i": O.OPS.index("SUM"), "r": O.OPS.index("SUM")}
+
+
+def test_reference_compute_prod_wrappers():
+    """The reference build's calReductionComputeProd3D{b,i,r} on the heat
+    model's fields equal its own calReductionOperation3D with PROD for the
+    byte variant and SUM for the int and real variants (the behaviour this
+    engine reproduces; the reference's reductions themselves cover only part
+    of the grid, so they are compared with each other, not with numpy)."""
+    R = _ref3()
+    if R is None:
+        pytest.skip("reference build oracle/_ref/libref_models3d.so not present")
+    R.gheat_compute_prod.restype = C.c_double
+    R.gheat_compute_prod.argtypes = [C.c_void_p, C.c_int]
+    R.gheat_load.argtypes = [C.c_void_p, _dp, _bp, _ip]
+    rng = np.random.default_rng(4)
+    shape = (3, 4, 5)
+    T0 = rng.uniform(0.5, 2.0, shape)
+    hot = np.ones(shape, np.uint8)
+    cnt = rng.integers(-50, 50, shape).astype(np.int32)
+    rec = np.zeros(8)
+    hd = R.gheat_create(4, 5, 3, T0.ctypes.data_as(_dp), 0.1, 50.0, -1e300, 1, rec.ctypes.data_as(_dp))
+    try:
+        assert R.gheat_load(hd, T0.ctypes.data_as(_dp), hot.ctypes.data_as(_bp), cnt.ctypes.data_as(_ip)) == 0
+        R.gheat_reduce.restype = C.c_double
+        R.gheat_reduce.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        SUM, PROD = O.OPS.index("SUM"), O.OPS.index("PROD")
+        for which in (0, 2):   # real T, int count
+            assert R.gheat_compute_prod(hd, which) == R.gheat_reduce(hd, which, SUM)
+        assert R.gheat_reduce(hd, 2, SUM) != R.gheat_reduce(hd, 2, PROD)
+        assert R.gheat_compute_prod(hd, 1) == R.gheat_reduce(hd, 1, PROD)
+    finally:
+        R.gheat_destroy(hd)
 
 
 def _check_reduction(got, vals, t, op):
@@ -517,6 +579,9 @@ def test_reductions_2d_gpu(shape):
                         got = E.gred_reduce(hd, which, op)
                         assert lib().calhipGetLastError() == 0, lib().calhipGetErrorString()
                         _check_reduction(got, arrays[which], t, op)
+                    # calReductionComputeProd2D*: PROD for bytes, SUM for int
+                    # and real, as the reference's wrappers
+                    _check_reduction(E.gred_compute_prod(hd, which), arrays[which], t, PROD_WRAPPER_OP[t])
         finally:
             E.gred_destroy(hd)
 
@@ -552,6 +617,7 @@ def test_reductions_3d_gpu():
                     got = E.gheat_reduce(hd, which, op)
                     assert lib().calhipGetLastError() == 0, lib().calhipGetErrorString()
                     _check_reduction(got, vals[which], t, op)
+                _check_reduction(E.gheat_compute_prod(hd, which), vals[which], t, PROD_WRAPPER_OP[t])
     finally:
         E.gheat_destroy(hd)
 
