@@ -96,8 +96,13 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024, batched=True):
     kavg = ktime_ms / max(launches, 1)
     return {
         "wall_s": wall, "gcups": cells * steps / wall / 1e9, "kernel_ms_avg": kavg, "kernel_launches": launches,
-        # algorithmic bytes (24 B per cell update) over the kernels' time
-        "achieved_gbs": BYTES_PER_CELL_SCIDDICAT * cells * steps / (ktime_ms * 1e-3) / 1e9,
+        # the bytes one launch must move at least (z and h read once, h
+        # written once: 24 B per cell, whether the launch advances one step
+        # or two) over the average launch time
+        "achieved_gbs": BYTES_PER_CELL_SCIDDICAT * cells / (kavg * 1e-3) / 1e9,
+        # 24 B per cell update over the same time: a bound only for a
+        # one-step launch (a two-step launch reads and writes once for two)
+        "per_update_gbs": BYTES_PER_CELL_SCIDDICAT * cells * steps / (ktime_ms * 1e-3) / 1e9,
         "steps_per_launch": steps / max(launches, 1),
         "launches_per_step": st["launches"] / max(st["steps"], 1), "mass_rel_err": mass_err,
         "uploads": st["uploads"],
@@ -118,6 +123,13 @@ def pcie_inclusive_sciddicat(n, steps, seed=2024):
     m.close()
     return {"gcups": round(n * n * steps / wall / 1e9, 3), "wall_s": round(wall, 4),
             "note": "calRun2D from host arrays incl. host init, upload of 6 substates x 2 layers and read-back"}
+
+
+def cpu_record(value, unit, kind, sample):
+    """A cpu_baseline object: value in unit, the host threads used (the
+    OpenMP setting the reference build runs with), what was timed."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(value, 6), "unit": unit, "cores": threads, "kind": kind, "sample": sample}
 
 
 def cpu_baseline_sciddicat(n, steps, seed=2024):
@@ -164,8 +176,9 @@ def bench_active(n=1024, steps=4000, seed=7, cpu_steps=400):
         t0 = time.perf_counter()
         native.ref_sciddicat(z, h, 0.5, 0.001, cpu_steps, True)
         cw = time.perf_counter() - t0
-        out["cpu_reference_ms_per_step"] = cw / cpu_steps * 1e3
-        out["cpu_sample"] = f"reference active-cells build, first {cpu_steps} steps incl. init"
+        out["cpu_baseline"] = cpu_record(cw / cpu_steps * 1e3, "ms/step", "reference",
+                                         f"reference OpenCAL-OMP active-cells sciddicaT {n}x{n}, first {cpu_steps} "
+                                         f"steps incl. init, {cw:.1f} s")
     return out
 
 
@@ -192,7 +205,14 @@ def bench_life(n=512, steps=1000, seed=1):
         res[mode] = (wall, got)
     want = native.oracle_life(a0, steps)
     alive = int(want.sum())
-    return {"workload": f"life_{n}x{n}_x{steps}_uint8",
+    cpu = None
+    if native.ref("life") is not None:
+        t0 = time.perf_counter()
+        native.ref_life(a0, steps)
+        cw = time.perf_counter() - t0
+        cpu = cpu_record(n * n * steps / cw / 1e9, "GCUPS", "reference",
+                         f"reference OpenCAL-OMP life (int substate) {n}x{n}, {steps} steps, {cw:.2f} s")
+    return {"workload": f"life_{n}x{n}_x{steps}_uint8", "cpu_baseline": cpu,
             "calRun2D_us_per_step": res["run"][0] / steps * 1e6,
             "calRun2D_gcups": n * n * steps / res["run"][0] / 1e9,
             "per_step_launch_us_per_step": res["steps"][0] / steps * 1e6,
@@ -212,26 +232,43 @@ def bench_batch(n_inst=256, steps=2000, seed=4):
     cells = sum(r * c for r, c, *_ in specs)
     log(f"batch: {n_inst} instances, {cells / 1e6:.1f} M cells")
     outs, ms = sciddicat_batch(insts, steps)
-    # the same instances one after another through the single-model C API
+    # the same instances one after another, each through calRun2D's step loop
+    # (calhipRunSteps2D: two steps per sweep, the headline's driver)
     seq_s = 0.0
     exact = True
     for (z, h, pr, pe), got in zip(insts, outs):
-        m = SciddicaT(z, h, pr, pe, steps, False)
+        m = SciddicaT(z, h, pr, pe, steps + 2, False)
         m.init()
         m.steps(1)
+        m.run_steps(2)   # upload, and the per-shape choice of the sweep, outside the timed steps
         m.sync()
         t0 = time.perf_counter()
-        m.steps(steps - 1)
+        m.run_steps(steps - 3)
         m.sync()
-        seq_s += (time.perf_counter() - t0) * steps / (steps - 1)
+        seq_s += (time.perf_counter() - t0) * steps / (steps - 3)
         _, hh, _ = m.read()
         m.close()
         exact = exact and bool(np.array_equal(hh, got))
     bat_gcups = cells * steps / (ms * 1e-3) / 1e9
     seq_gcups = cells * steps / seq_s / 1e9
+    # CPU: the reference build on a sample of the instances, 100 steps each
+    cpu = None
+    import native
+    if native.ref("sciddicat") is not None:
+        sample = insts[:: max(1, len(insts) // 8)][:8]
+        sc = sum(z.size for z, *_ in sample)
+        t0 = time.perf_counter()
+        for z, h, pr, pe in sample:
+            native.ref_sciddicat(z, h, pr, pe, 100)
+        cw = time.perf_counter() - t0
+        cpu = cpu_record(sc * 100 / cw / 1e9, "GCUPS", "reference",
+                         f"reference OpenCAL-OMP sciddicaT, {len(sample)} of the instances ({sc / 1e6:.1f} M cells) "
+                         f"one after another, 100 steps each incl. init, {cw:.1f} s")
     return {"workload": f"sciddicaT_batch_{n_inst}_instances_128..1024_x{steps}", "cells": cells,
             "batched_gcups": bat_gcups, "sequential_gcups": seq_gcups, "speedup": bat_gcups / seq_gcups,
-            "batched_s": ms * 1e-3, "sequential_s": seq_s, "each_instance_equals_its_single_run": exact}
+            "sequential_driver": "each instance through calhipRunSteps2D (calRun2D's loop, two steps per sweep)",
+            "batched_s": ms * 1e-3, "sequential_s": seq_s, "each_instance_equals_its_single_run": exact,
+            "cpu_baseline": cpu}
 
 
 def bench_adjoint(n=1024, steps=2000, ck=45, seed=5):
@@ -249,6 +286,21 @@ def bench_adjoint(n=1024, steps=2000, ck=45, seed=5):
     forward_only(z_dem, patch0, (r0, c0), 0.5, 0.001, steps)
     torch.cuda.synchronize()
     fwd_s = time.perf_counter() - t0
+    # the best plain run of the same forward: the C API model through
+    # calRun2D's loop (two steps per sweep)
+    from opencal_amd.models import SciddicaT
+    h_src = np.zeros_like(z_np)
+    h_src[r0:r0 + p, c0:c0 + p] = h_np[r0:r0 + p, c0:c0 + p]
+    m = SciddicaT(z_np, h_src, 0.5, 0.001, steps + 2, False)
+    m.init()
+    m.steps(1)
+    m.run_steps(2)   # upload and the per-shape sweep choice outside the timed steps
+    m.sync()
+    t0 = time.perf_counter()
+    m.run_steps(steps - 3)
+    m.sync()
+    plain_s = (time.perf_counter() - t0) * steps / (steps - 3)
+    m.close()
     def grad_run():
         pr = torch.tensor(0.5, dtype=torch.float64, requires_grad=True)
         pe = torch.tensor(0.001, dtype=torch.float64, requires_grad=True)
@@ -271,7 +323,9 @@ def bench_adjoint(n=1024, steps=2000, ck=45, seed=5):
     fd = (L(0.5 + h) - L(0.5 - h)) / (2 * h)
     g = pr.grad.item()
     return {"workload": f"sciddicaT_adjoint_{n}x{n}_x{steps}_ck{ck}", "forward_s": fwd_s,
-            "gradient_s": grad_s, "gradient_over_forward": grad_s / fwd_s, "loss": loss.item(),
+            "gradient_s": grad_s, "gradient_over_forward": grad_s / fwd_s,
+            "plain_calRun2D_forward_s": plain_s, "gradient_over_plain_forward": grad_s / plain_s,
+            "loss": loss.item(),
             "dL_dpr": g, "dL_dpeps": pe.grad.item(), "patch_grad_norm": patch.grad.norm().item(),
             "fd_dL_dpr": fd, "fd_rel_err": abs(g - fd) / max(abs(fd), 1e-30)}
 
@@ -306,15 +360,21 @@ def bench_sciara(n=2048, seed=3, cpu_steps=200):
     pac = inp["cell_size"] ** 2
     em = float((inp["rates"] * inp["params"][0] / pac).sum() * (inp["emission_time"] / inp["params"][0]))
     out["volume_balance_rel"] = float((res["h"].sum() + res["Msl"].sum() - em) / em)
-    # CPU: the oracle restatement (libm, as the reference) on the first
-    # cpu_steps steps of the same run; its cost per step is dominated by the
-    # full-grid active-set rebuilds and barely depends on the lava extent
-    nt = native.set_threads(int(os.environ.get("OMP_NUM_THREADS", "16")))
+    # CPU: the reference build (Sciara.cpp on the OpenCAL-OMP engine) on the
+    # first cpu_steps steps of the same run, else the oracle restatement; the
+    # cost per step is dominated by the full-grid active-set rebuilds and
+    # barely depends on the lava extent
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    kind = "reference" if native.ref("sciara") is not None else "port"
     t0 = time.perf_counter()
-    native.oracle_sciara(inp, cpu_steps, True, crmath=False)
+    if kind == "reference":
+        native.ref_sciara(inp, cpu_steps, True)
+    else:
+        native.oracle_sciara(inp, cpu_steps, True, crmath=False)
     cw = time.perf_counter() - t0
-    out["cpu_port_ms_per_step"] = cw / cpu_steps * 1e3
-    out["cpu_sample"] = f"oracle (libm mode) first {cpu_steps} steps incl. init, {nt} threads"
+    out["cpu_baseline"] = cpu_record(cw / cpu_steps * 1e3, "ms/step", kind,
+                                     f"{'reference OpenCAL-OMP Sciara.cpp' if kind == 'reference' else 'oracle'} "
+                                     f"{n}x{n} active cells, first {cpu_steps} steps incl. init, {cw:.1f} s")
     return out
 
 
@@ -371,11 +431,14 @@ def main():
                                        "WRITE_SIZE, per two-step launch)",
                      "kernel": "step_fused_2x", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
                      "steps_per_launch": round(r["steps_per_launch"], 4),
-                     "bytes_per_cell_update": BYTES_PER_CELL_SCIDDICAT,
-                     "note": "achieved = 24 B per cell update x cell updates / kernel time (the contract's "
-                             "algorithmic bytes); a two-step launch moves about 24 B per cell for two updates, "
-                             "so its HBM traffic (traffic, per launch) is about half of those algorithmic bytes; "
-                             "traffic_gbs / traffic_frac = that measured traffic over the same kernel time"},
+                     "algorithmic_bytes_per_launch": int(BYTES_PER_CELL_SCIDDICAT * args.size * args.size),
+                     "per_update_gbs": round(r["per_update_gbs"], 1),
+                     "note": "achieved = the bytes one launch must move (z and h read once, h written once: "
+                             "24 B per cell; the launch advances two steps and keeps the intermediate level in "
+                             "registers) / the launch's average duration from HIP events on the model's stream; "
+                             "per_update_gbs = 24 B per cell update over the same time, not a bound for a "
+                             "launch that advances two steps; traffic = measured HBM bytes per launch "
+                             "(rocprofv3 counters), traffic_gbs / traffic_frac over the same time"},
         "per_step_driver": {
             "driver": "calRunCAStep2D per step (one sweep per step)", "gcups": round(r1["gcups"], 3),
             "ms_per_step": round(r1["wall_s"] / args.steps * 1e3, 4),
@@ -383,10 +446,12 @@ def main():
                          "unit": "GB/s", "frac": round(r1["achieved_gbs"] / HBM_PEAK_GBS, 4),
                          "traffic": profiled_traffic() if args.size == 4096 else None,
                          "traffic_source": "profiles/sciddicat_sweep_4096x4096.md",
-                         "kernel": "step_fused_al", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2)}},
+                         "kernel": "step_fused_al", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2),
+                         "algorithmic_bytes_per_launch": int(BYTES_PER_CELL_SCIDDICAT * args.size * args.size)}},
     }
     # the measured HBM bytes per launch over the same kernel time: the rate the
     # memory actually sustains, beside the algorithmic rate of `achieved`
+    # (traffic above the algorithmic bytes = re-read halo rows)
     for rl in (out["roofline"], out["per_step_driver"]["roofline"]):
         if rl["traffic"]:
             rl["traffic_gbs"] = round(rl["traffic"] / (rl["kernel_us"] * 1e-6) / 1e9, 1)

@@ -453,7 +453,8 @@ struct SciddicaTProgram : Program {
 	double* d_trash = nullptr;   // sink for the stores of halo lanes
 	SweepShape shape = {0, 0};   // chunk length and prefetch depth (0: not chosen yet)
 	WetOrder wet_order;          // wet-first tile order of the fused sweep
-	Sweep2Shape shape2 = {0, 0}; // tiles of the two-step sweep (0: not chosen yet)
+	Sweep2Shape shape2 = {0, 0}; // tiles of the two-step sweep ({0, 0}: one step per launch is faster)
+	bool shape2_known = false;   // shape2 chosen
 	WetOrder wet_order2;         // its wet-first order
 
 	// flag-driven active step: tile lists by parity (one allocation from
@@ -675,7 +676,8 @@ struct SciddicaTProgram : Program {
 
 	// Full grid with the steering twin: pairs of steps go to the two-step
 	// sweep (the steering's reset between them is what makes a step a
-	// function of z and h alone); the rest, one step at a time.
+	// function of z and h alone) when the tuner found it faster than two
+	// one-step launches on this grid; the rest, one step at a time.
 	int run_steps(ModelX* m, int n, bool with_steering) override
 	{
 		const int rows = m->pub.rows, cols = m->pub.columns;
@@ -688,9 +690,12 @@ struct SciddicaTProgram : Program {
 			if (!zero) break;
 			const double r = *p_r, eps = *p_eps;
 			if (!d_trash && CALHIP_CHECK(hipMalloc(&d_trash, 128 * sizeof(double)))) return -1;
-			if (shape2.s_rows == 0)
+			if (!shape2_known) {
 				shape2 = sweep2_shape(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,
 				                      (double*)h->d_nxt, d_trash, r, eps);
+				shape2_known = true;
+			}
+			if (shape2.s_rows == 0) break;   // one step per launch is faster on this grid
 			hipEvent_t ev;
 			timer_begin(m, &ev);
 			launch_sweep2(m->stream, rows, cols, (const double*)z->d_cur, (const double*)h->d_cur,

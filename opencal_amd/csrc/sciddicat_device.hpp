@@ -655,8 +655,11 @@ SweepShape sweep_shape(hipStream_t st, int rows, int cols, const double* z, cons
 SweepShape sweep_shape_cached(int rows, int cols);
 // Two steps per sweep (sciddicat_2step.hpp, flows reset between the steps):
 // the grids it supports (even cols >= 8, rows >= 8), one launch z, h -> hn,
-// and its tile shape (CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X = small-tile rows,
-// CALHIP_SCIDDICAT_2X_GROUP = small tiles per big tile, else 12 and 4).
+// and its tile shape: CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X (small-tile rows) /
+// CALHIP_SCIDDICAT_2X_GROUP (small tiles per big tile), else the shape the
+// tuner times fastest per grid shape, or {0, 0} when two launches of the
+// one-step sweep beat every two-step shape (cached per grid shape; the
+// timing sweeps write into `scratch`).
 struct Sweep2Shape {
 	int s_rows, bigk;   // small-tile rows, small tiles per big tile
 };

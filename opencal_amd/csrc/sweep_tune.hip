@@ -305,21 +305,71 @@ SweepShape sweep_shape_cached(int rows, int cols)
 	return it != tuned.end() ? it->second : SweepShape{16, 1};
 }
 
+namespace {
+std::mutex tune2_mu;
+std::map<std::pair<int, int>, Sweep2Shape> tuned2;
+}  // namespace
+
 Sweep2Shape sweep2_shape(hipStream_t st, int rows, int cols, const double* z, const double* h, double* scratch,
                          double* trash, double r, double eps)
 {
-	(void)st, (void)rows, (void)cols, (void)z, (void)h, (void)scratch, (void)trash, (void)r, (void)eps;
-	// Small tiles of 12 rows, big tiles of four: measured at 4096^2 (full
-	// grid, 500 steps of a spreading front) against 6..24-row small tiles and
-	// groups of 2..8 (12 x 4: 101 us per launch; 12 x 3: 107; 8 x 4: 103;
-	// 16 x 2: 112).  Not autotuned: the best shape depends on how much of the
-	// grid is wet, which changes as the run advances.
-	Sweep2Shape s = {12, 4};
+	// The environment fixes the tile shape (and so the two-step sweep).
+	Sweep2Shape fixed = {0, 0};
 	if (const char* e = getenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X"))
-		if (atoi(e) > 0) s.s_rows = atoi(e);
+		if (atoi(e) > 0) fixed.s_rows = atoi(e);
 	if (const char* e = getenv("CALHIP_SCIDDICAT_2X_GROUP"))
-		if (atoi(e) > 0) s.bigk = atoi(e);
-	return s;
+		if (atoi(e) > 0) fixed.bigk = atoi(e);
+	if (fixed.s_rows > 0 || fixed.bigk > 0) {
+		if (fixed.s_rows <= 0) fixed.s_rows = 12;
+		if (fixed.bigk <= 0) fixed.bigk = 4;
+		return fixed;
+	}
+	// The one-step sweep's shape is tuned first (it has its own lock).
+	const SweepShape one = sweep_shape(st, rows, cols, z, h, scratch, trash, r, eps);
+	std::lock_guard<std::mutex> lk(tune2_mu);
+	const auto key = std::make_pair(rows, cols);
+	auto it = tuned2.find(key);
+	if (it != tuned2.end()) return it->second;
+	// Small tiles of s_rows rows in big tiles of bigk: at 4096^2 with a
+	// spreading front 12 x 4 ran 101 us per launch (12 x 3: 107; 8 x 4: 103;
+	// 16 x 2: 112); on small grids the big tiles are too few to fill the
+	// chip, and shorter tiles, or one step per launch, win.  Each candidate
+	// is timed (its own wet-first state, one launch to learn the wet tiles,
+	// then three) against two launches of the tuned one-step sweep; {0, 0}
+	// means one step per launch.
+	static const Sweep2Shape cand[] = {{12, 4}, {12, 2}, {12, 1}, {8, 4}, {8, 2}, {8, 1}, {16, 2}, {6, 2}, {24, 1}};
+	Sweep2Shape best = {12, 4};
+	float best_ms = 0.f;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+		for (Sweep2Shape c : cand) {
+			WetOrder wo;
+			launch_sweep2(st, rows, cols, z, h, scratch, trash, r, eps, c, &wo);
+			(void)hipEventRecord(e0, st);
+			for (int k = 0; k < 3; ++k) launch_sweep2(st, rows, cols, z, h, scratch, trash, r, eps, c, &wo);
+			(void)hipEventRecord(e1, st);
+			float ms = 0.f;
+			if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+			if (best_ms == 0.f || ms < best_ms) {
+				best_ms = ms;
+				best = c;
+			}
+		}
+		WetOrder wo;
+		launch_sweep(st, rows, cols, z, h, scratch, trash, r, eps, one, &wo);
+		(void)hipEventRecord(e0, st);
+		for (int k = 0; k < 6; ++k) launch_sweep(st, rows, cols, z, h, scratch, trash, r, eps, one, &wo);
+		(void)hipEventRecord(e1, st);
+		float ms1 = 0.f;
+		if (best_ms > 0.f && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms1, e0, e1) == hipSuccess &&
+		    ms1 < best_ms)
+			best = Sweep2Shape{0, 0};   // six one-step launches (six steps) beat three two-step launches
+	}
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	(void)hipGetLastError();
+	tuned2[key] = best;
+	return best;
 }
 
 }  // namespace sct

@@ -19,9 +19,13 @@ from opencal_amd import synth
 pytestmark = pytest.mark.gpu
 
 
-def test_sciddicat_4096_two_step_loop_vs_oracle():
+def test_sciddicat_4096_two_step_loop_vs_oracle(monkeypatch):
     from opencal_amd.models import SciddicaT
     print("case config 4096x4096 full grid, 100 steps of calhipRunSteps2D", flush=True)
+    # the benchmark's tile shape, fixed (the tuner also picks the two-step
+    # sweep at this size)
+    monkeypatch.setenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X", "12")
+    monkeypatch.setenv("CALHIP_SCIDDICAT_2X_GROUP", "4")
     z, h, _ = synth.sciddicat_inputs(4096, 4096, 2024)
     m = SciddicaT(z, h, 0.5, 0.001, 100, False)
     try:

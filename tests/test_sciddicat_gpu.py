@@ -286,3 +286,25 @@ def test_two_step_sweep(rows, cols, s_rows, group, monkeypatch):
     _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 21, False)
     assert np.array_equal(gh, oh), f"mixed: max |dh| = {np.abs(gh - oh).max()}"
     assert st["steps"] == 21
+
+
+@pytest.mark.parametrize("rows,cols", [(1024, 1024), (300, 384)])
+def test_run_steps_tuned_choice(rows, cols, monkeypatch):
+    """calhipRunSteps2D with the tuner's own choice for the grid (a two-step
+    tile shape, or one step per launch when that is faster): bit-exact
+    against the oracle."""
+    from opencal_amd.models import SciddicaT
+    print(f"case run_steps tuned choice {rows}x{cols}", flush=True)
+    monkeypatch.delenv("CALHIP_SCIDDICAT_ROWS_PER_WAVE_2X", raising=False)
+    monkeypatch.delenv("CALHIP_SCIDDICAT_2X_GROUP", raising=False)
+    z, h, _ = synth.sciddicat_inputs(rows, cols, rows + 3 * cols)
+    m = SciddicaT(z, h, 0.5, 0.001, 64, active=False)
+    try:
+        m.init()
+        m.steps(1)
+        m.run_steps(63)
+        _, gh, _ = m.read()
+    finally:
+        m.close()
+    _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 64, False)
+    assert np.array_equal(gh, oh), f"max |dh| = {np.abs(gh - oh).max()}"
