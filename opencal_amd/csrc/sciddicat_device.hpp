@@ -190,7 +190,8 @@ constexpr int FUSED_WAVES = 4;   // waves per block
 constexpr int STRIP2_OUT = 124;
 
 // One wave sweeps rows [r0, r1) of the 128-column strip `strip` of a
-// rows x cols toroidal grid.  EVEN: cols is even, so a lane's two cells are
+// rows x cols toroidal grid and returns (wave-uniform) whether any cell whose
+// outflows it computed (the strip and its halo) was wet.  EVEN: cols is even, so a lane's two cells are
 // one aligned 16-byte access; otherwise each cell is loaded and stored on its
 // own (the pair may straddle the wrap).  UP: walk the chunk bottom-up.
 // Neighbouring chunks walk in opposite directions, so the halo rows they share
@@ -201,7 +202,7 @@ constexpr int STRIP2_OUT = 124;
 // positions q..q+2 and writes row(q) for q >= 2 (its flows, those of the row
 // behind it, q-1, and those of the row ahead of it, q+1, are then known).
 template <int D, bool EVEN, bool UP>
-__device__ __forceinline__ void sweep_strip(const double* __restrict__ z, const double* __restrict__ h,
+__device__ __forceinline__ bool sweep_strip(const double* __restrict__ z, const double* __restrict__ h,
                                             double* __restrict__ hn, double* __restrict__ trash, int rows, int cols,
                                             int strip, int r0, int r1, double r, double eps)
 {
@@ -239,6 +240,7 @@ __device__ __forceinline__ void sweep_strip(const double* __restrict__ z, const 
 	// the outflow towards row(q) (S outflow walking down, N outflow walking up)
 	double pa0 = 0.0, pa1 = 0.0, pa2 = 0.0, pa3 = 0.0, pb0 = 0.0, pb1 = 0.0, pb2 = 0.0, pb3 = 0.0;
 	double ppa = 0.0, ppb = 0.0;
+	bool seen = false;
 
 	for (int base = 0; base < n_it; base += W) {
 #pragma unroll
@@ -252,6 +254,7 @@ __device__ __forceinline__ void sweep_strip(const double* __restrict__ z, const 
 			const double2 zb = bz[sb], hb = bh[sb], zc = bz[sc], hc = bh[sc], za = bz[sa], ha = bh[sa];
 			const double2 zN = UP ? za : zb, hN = UP ? ha : hb, zS = UP ? zb : za, hS = UP ? hb : ha;
 			const double ua = zc.x + hc.x, ub = zc.y + hc.y;
+			seen = seen || !(hc.x <= eps) || !(hc.y <= eps);
 			const double uw = dev::from_left(ub);    // west of a: right element of lane-1
 			const double ue = dev::from_right(ua);   // east of b: left element of lane+1
 			PairFlows pf;
@@ -299,18 +302,17 @@ __device__ __forceinline__ void sweep_strip(const double* __restrict__ z, const 
 			__builtin_amdgcn_sched_barrier(0);
 		}
 	}
+	return __any((int)seen) != 0;
 }
 
 // Both walking directions; chunks alternate.
 template <int D, bool EVEN>
-__device__ __forceinline__ void sweep_chunk(const double* __restrict__ z, const double* __restrict__ h,
+__device__ __forceinline__ bool sweep_chunk(const double* __restrict__ z, const double* __restrict__ h,
                                             double* __restrict__ hn, double* __restrict__ trash, int rows, int cols,
                                             int strip, int chunk, int r0, int r1, double r, double eps)
 {
-	if (chunk & 1)
-		sweep_strip<D, EVEN, true>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
-	else
-		sweep_strip<D, EVEN, false>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
+	if (chunk & 1) return sweep_strip<D, EVEN, true>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
+	return sweep_strip<D, EVEN, false>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
 }
 
 // ---------------------------------------------------------------------------
