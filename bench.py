@@ -221,7 +221,7 @@ def bench_life(n=512, steps=1000, seed=1):
             "note": "wall time incl. upload/download of the 256 KiB board"}
 
 
-def bench_batch(n_inst=256, steps=2000, seed=4):
+def bench_batch(n_inst=256, steps=2000, seed=4, sequential=True):
     from opencal_amd import synth
     from opencal_amd.models import SciddicaT, sciddicat_batch
     specs = synth.batch_instances(n_inst, seed)
@@ -232,6 +232,9 @@ def bench_batch(n_inst=256, steps=2000, seed=4):
     cells = sum(r * c for r, c, *_ in specs)
     log(f"batch: {n_inst} instances, {cells / 1e6:.1f} M cells")
     outs, ms = sciddicat_batch(insts, steps)
+    if not sequential:
+        return {"workload": f"sciddicaT_batch_{n_inst}_instances_128..1024_x{steps}", "cells": cells,
+                "batched_gcups": cells * steps / (ms * 1e-3) / 1e9, "batched_s": ms * 1e-3}
     # the same instances one after another, each through calRun2D's step loop
     # (calhipRunSteps2D: two steps per sweep, the headline's driver)
     seq_s = 0.0
@@ -389,6 +392,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--only", choices=["active", "life", "batch", "adjoint", "sciara"],
                     help="run one secondary workload alone and print its result (for profiling)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="with --only batch: the batched run alone, without the sequential and CPU legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on one MI355X; "
@@ -400,7 +405,10 @@ def main():
     workloads = {"active": bench_active, "life": bench_life, "batch": bench_batch, "adjoint": bench_adjoint,
                  "sciara": bench_sciara}
     if args.only:
-        print(json.dumps({args.only: workloads[args.only]()}), flush=True)
+        fn = workloads[args.only]
+        if args.only == "batch" and args.batched_only:
+            fn = lambda: bench_batch(sequential=False)   # noqa: E731
+        print(json.dumps({args.only: fn()}), flush=True)
         return
     log(f"sciddicaT plain {args.size}^2: warmup {args.warmup}, timed {args.steps}")
     r = bench_sciddicat_plain(args.size, args.steps, args.warmup, batched=True)

@@ -24,9 +24,10 @@
 // steps 0 and 1 sweep every tile.  Skipped tiles keep bits that equal the
 // full sweep's.
 //
-// Launches.  One launch per step: its waves pull tiles of the step's queue
-// with an atomic counter until it is empty, and the launch boundary orders
-// the step's writes (levels, next queue) before the next step.  Three queues
+// Launches.  The steps run in one cooperative launch: its waves pull tiles
+// of the step's queue with an atomic counter until it is empty, and a grid
+// barrier orders the step's writes (levels, next queue) before the next
+// step.  Without cooperative launches each step is one launch.  Three queues
 // rotate: step t reads queue t%3, appends to (t+1)%3 and clears (t+2)%3,
 // which nothing reads or appends to during step t.
 //
@@ -40,6 +41,8 @@
 #include <map>
 #include <mutex>
 #include <vector>
+
+#include <hip/hip_cooperative_groups.h>
 
 #include "engine.hpp"
 #include "device_common.hpp"
@@ -79,9 +82,11 @@ __device__ __forceinline__ void queue_tile(const BatchQueues& Q, int tile, int s
 // the pull (lane 0 adds 1, the others 0), so no lane can leave the loop or
 // re-enter it apart from the others.
 template <int D>
-__global__ __launch_bounds__(BATCH_BLOCK) __attribute__((amdgpu_waves_per_eu(D == 1 ? 3 : 2))) void batch_step(
-    const BatchInst* __restrict__ inst, const int* __restrict__ tile_inst, int rpw, const double* __restrict__ z,
-    double* __restrict__ h0, double* __restrict__ h1, double* __restrict__ trash, BatchQueues Q, int t)
+__device__ __forceinline__ void batch_step_body(const BatchInst* __restrict__ inst,
+                                                const int* __restrict__ tile_inst, int rpw,
+                                                const double* __restrict__ z, double* __restrict__ h0,
+                                                double* __restrict__ h1, double* __restrict__ trash, BatchQueues Q,
+                                                int t)
 {
 	const int lane = threadIdx.x & 63;
 	const int qi = t % 3;
@@ -123,6 +128,31 @@ __global__ __launch_bounds__(BATCH_BLOCK) __attribute__((amdgpu_waves_per_eu(D =
 				queue_tile(Q, I.first + chunk * I.strips + se, t + 1);
 			}
 		}
+	}
+}
+
+
+// One step per launch (the launch boundary orders the step's writes before
+// the next step).
+template <int D>
+__global__ __launch_bounds__(BATCH_BLOCK) __attribute__((amdgpu_waves_per_eu(D == 1 ? 3 : 2))) void batch_step(
+    const BatchInst* __restrict__ inst, const int* __restrict__ tile_inst, int rpw, const double* __restrict__ z,
+    double* __restrict__ h0, double* __restrict__ h1, double* __restrict__ trash, BatchQueues Q, int t)
+{
+	batch_step_body<D>(inst, tile_inst, rpw, z, h0, h1, trash, Q, t);
+}
+
+// Steps [t0, t1) in one cooperative launch: every block is resident, and a
+// grid barrier (with its device-scope release and acquire) separates the
+// steps.  Every wave leaves each step's loop and reaches every barrier.
+template <int D>
+__global__ __launch_bounds__(BATCH_BLOCK) __attribute__((amdgpu_waves_per_eu(D == 1 ? 3 : 2))) void batch_steps_coop(
+    const BatchInst* __restrict__ inst, const int* __restrict__ tile_inst, int rpw, const double* __restrict__ z,
+    double* __restrict__ h0, double* __restrict__ h1, double* __restrict__ trash, BatchQueues Q, int t0, int t1)
+{
+	for (int t = t0; t < t1; ++t) {
+		batch_step_body<D>(inst, tile_inst, rpw, z, h0, h1, trash, Q, t);
+		cooperative_groups::this_grid().sync();
 	}
 }
 
@@ -205,7 +235,7 @@ struct Runner {
 
 	// `steps` steps with chunk length rpw; the stepping time in ms (< 0 on
 	// error); with h_out the final thickness is copied out.
-	float run(int rpw, int depth, int steps, double* const* h_out)
+	float run(int rpw, int depth, int steps, int per_launch, double* const* h_out)
 	{
 		const Layout L = make_layout(n, rows, cols, p_r, p_eps, rpw);
 		const int n_tiles = (int)L.tile_inst.size();
@@ -238,7 +268,7 @@ struct Runner {
 			                                stream)))
 				break;
 			if (CALHIP_CHECK(hipEventCreate(&e0)) || CALHIP_CHECK(hipEventCreate(&e1))) break;
-			if (launch_all(rpw, depth, steps, (const double*)dz.p, (double*)dh0.p, (double*)dh1.p, (double*)dtrash.p,
+			if (launch_all(rpw, depth, steps, per_launch, (const double*)dz.p, (double*)dh0.p, (double*)dh1.p, (double*)dtrash.p,
 			               (const BatchInst*)dinst.p, (const int*)dti.p, Q, e0, e1))
 				break;
 			std::vector<double> hh;
@@ -264,25 +294,53 @@ struct Runner {
 		return ms;
 	}
 
-	// The stepping launches between events e0 and e1, one per step.
-	int launch_all(int rpw, int depth, int steps, const double* z, double* h0, double* h1, double* trash,
-	               const BatchInst* dinst, const int* dti, BatchQueues Q, hipEvent_t e0, hipEvent_t e1)
+	// The stepping launches between events e0 and e1: `per_launch` steps per
+	// cooperative launch (0: all of them), or one ordinary launch per step
+	// (per_launch 1, or when the device offers no cooperative launch).
+	int launch_all(int rpw, int depth, int steps, int per_launch, const double* z, double* h0, double* h1,
+	               double* trash, const BatchInst* dinst, const int* dti, BatchQueues Q, hipEvent_t e0, hipEvent_t e1)
 	{
-		int dev = 0, cus = 0;
+		int dev = 0, cus = 0, coop = 0, per_cu = 0;
 		if (CALHIP_CHECK(hipGetDevice(&dev)) ||
-		    CALHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)))
+		    CALHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) ||
+		    CALHIP_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev)))
 			return -1;
-		const int grid = 3 * cus;   // three blocks per CU (the sweep's occupancy at prefetch depth 1)
+		const void* kcoop = depth >= 2 ? (const void*)batch_steps_coop<2> : (const void*)batch_steps_coop<1>;
+		if (coop && per_launch != 1 &&
+		    CALHIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kcoop, BATCH_BLOCK, 0)))
+			return -1;
 		if (CALHIP_CHECK(hipEventRecord(e0, stream))) return -1;
-		for (int t = 0; t < steps; ++t) {
-			if (depth >= 2)
-				hipLaunchKernelGGL(batch_step<2>, dim3(grid), dim3(BATCH_BLOCK), 0, stream, dinst, dti, rpw, z, h0, h1,
-				                   trash, Q, t);
-			else
-				hipLaunchKernelGGL(batch_step<1>, dim3(grid), dim3(BATCH_BLOCK), 0, stream, dinst, dti, rpw, z, h0, h1,
-				                   trash, Q, t);
+		if (coop && per_launch != 1 && per_cu > 0) {
+			// exactly the blocks that fit at once: the launch is refused
+			// rather than started when they do not
+			// two blocks per CU: 256 instances x 2000 steps took 772 ms against
+			// 935 (one), 845 (three, the most that fit) and 799 with one
+			// launch per step; fewer blocks meet at each barrier, more hide
+			// the sweep's memory latency
+			int bpc = std::min(per_cu, 2);
+			if (const char* e = getenv("CALHIP_BATCH_COOP_BLOCKS_PER_CU"))
+				if (atoi(e) > 0 && atoi(e) <= per_cu) bpc = atoi(e);
+			const int grid = bpc * cus;
+			const int chunk = per_launch > 0 ? per_launch : steps;
+			for (int t0 = 0; t0 < steps; t0 += chunk) {
+				int t1 = std::min(steps, t0 + chunk);
+				void* args[] = {(void*)&dinst, (void*)&dti, (void*)&rpw, (void*)&z, (void*)&h0,
+				                (void*)&h1,    (void*)&trash, (void*)&Q, (void*)&t0, (void*)&t1};
+				if (CALHIP_CHECK(hipLaunchCooperativeKernel(kcoop, dim3(grid), dim3(BATCH_BLOCK), args, 0, stream)))
+					return -1;
+			}
+		} else {
+			const int grid = 3 * cus;   // three blocks per CU (the sweep's occupancy at prefetch depth 1)
+			for (int t = 0; t < steps; ++t) {
+				if (depth >= 2)
+					hipLaunchKernelGGL(batch_step<2>, dim3(grid), dim3(BATCH_BLOCK), 0, stream, dinst, dti, rpw, z, h0,
+					                   h1, trash, Q, t);
+				else
+					hipLaunchKernelGGL(batch_step<1>, dim3(grid), dim3(BATCH_BLOCK), 0, stream, dinst, dti, rpw, z, h0,
+					                   h1, trash, Q, t);
+			}
+			if (CALHIP_CHECK(hipGetLastError())) return -1;
 		}
-		if (CALHIP_CHECK(hipGetLastError())) return -1;
 		return CALHIP_CHECK(hipEventRecord(e1, stream));
 	}
 };
@@ -314,9 +372,10 @@ extern "C" int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols, 
 			set_error(41, "calhipSciddicaTBatchRun: every instance needs a grid of at least 3 x 3 and its arrays");
 			return -1;
 		}
-	int depth = 1, rpw = 0;
+	int depth = 1, rpw = 0, per_launch = 0;
 	if (const char* e = getenv("CALHIP_BATCH_ROWS_PER_WAVE")) rpw = atoi(e) > 0 ? atoi(e) : 0;
 	if (const char* e = getenv("CALHIP_BATCH_PREFETCH")) depth = atoi(e) > 0 ? atoi(e) : depth;
+	if (const char* e = getenv("CALHIP_BATCH_STEPS_PER_LAUNCH")) per_launch = atoi(e) > 0 ? atoi(e) : 0;
 	Runner R{n, rows, cols, p_r, p_eps};
 	if (CALHIP_CHECK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking))) return -1;
 	int rc = -1;
@@ -333,7 +392,7 @@ extern "C" int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols, 
 				static const int cand[] = {8, 16, 32};
 				float best = -1.f;
 				for (int c : cand) {
-					const float ms = R.run(c, depth, std::min(steps, TUNE_STEPS), nullptr);
+					const float ms = R.run(c, depth, std::min(steps, TUNE_STEPS), per_launch, nullptr);
 					if (ms < 0.f) break;
 					if (best < 0.f || ms < best) {
 						best = ms;
@@ -344,7 +403,7 @@ extern "C" int calhipSciddicaTBatchRun(int n, const int* rows, const int* cols, 
 				tuned_rpw[key] = rpw;
 			}
 		}
-		const float ms = R.run(rpw, depth, steps, h_out);
+		const float ms = R.run(rpw, depth, steps, per_launch, h_out);
 		if (ms < 0.f) break;
 		if (timing_ms) *timing_ms = ms;
 		rc = 0;

@@ -2,7 +2,8 @@
 (bit-exact), against the oracle and against the single-model GPU path;
 uneven sizes including odd column counts and grids narrower than a strip;
 the wet-tile queue (dry tiles skipped from step 2 on) under several tile
-row counts, including runs of 2 and 3 steps (the queue's first steps)."""
+row counts and launch forms, including runs of 2 and 3 steps (the queue's
+first steps)."""
 import numpy as np
 import pytest
 
@@ -12,19 +13,25 @@ from opencal_amd import synth
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("steps,rpw", [(150, "32"), (151, "8"), (151, "16"), (2, "16"), (3, "32")])
-def test_batch_vs_oracle(steps, rpw, monkeypatch):
+@pytest.mark.parametrize("steps,rpw,per_launch,bpc", [(150, "32", "0", "0"), (151, "8", "0", "3"), (151, "16", "1", "0"),
+                                                     (150, "8", "7", "1"), (2, "16", "0", "0"), (3, "32", "1", "0")])
+def test_batch_vs_oracle(steps, rpw, per_launch, bpc, monkeypatch):
     """Uneven sizes, including grids shorter than a chunk and narrower than
-    a strip, over an even and an odd number of steps."""
+    a strip, over an even and an odd number of steps; all steps in one
+    cooperative launch (two or three blocks per CU), launches of 7 steps
+    (one block per CU), or one launch per step."""
     from opencal_amd.models import sciddicat_batch
     monkeypatch.setenv("CALHIP_BATCH_ROWS_PER_WAVE", rpw)
+    monkeypatch.setenv("CALHIP_BATCH_STEPS_PER_LAUNCH", per_launch)
+    monkeypatch.setenv("CALHIP_BATCH_COOP_BLOCKS_PER_CU", bpc)
     specs = synth.batch_instances(12, seed=77, lo=24, hi=180)
     specs += [(6, 40, 0.5, 0.002, 901), (20, 6, 0.45, 0.001, 902), (9, 8, 0.6, 0.004, 903), (8, 130, 0.5, 0.001, 904)]
     insts = []
     for rows, cols, pr, pe, seed in specs:
         z, h, _ = synth.sciddicat_inputs(rows, cols, seed, patch=min(12, rows // 4 or 1))
         insts.append((z, h, pr, pe))
-    print("case batch steps", steps, "rows", rpw, "sizes", [(r, c) for r, c, *_ in specs], flush=True)
+    print("case batch steps", steps, "rows", rpw, "per launch", per_launch, "blocks/CU", bpc,
+          "sizes", [(r, c) for r, c, *_ in specs], flush=True)
     outs, ms = sciddicat_batch(insts, steps)
     assert ms > 0
     for (z, h, pr, pe), got in zip(insts, outs):
