@@ -314,11 +314,12 @@ struct Runner {
 			// exactly the blocks that fit at once: the launch is refused
 			// rather than started when they do not
 			// two blocks per CU: 256 instances x 2000 steps took 772 ms against
-			// 935 (one), 845 (three, the most that fit) and 799 with one
-			// launch per step; fewer blocks meet at each barrier, more hide
-			// the sweep's memory latency
+			// 935 (one) and 845 (three, the most that fit); one launch per
+			// step on the same grid takes the same 771 ms (an idle grid
+			// barrier costs 34-78 us here against 2.5-2.8 us per empty
+			// launch, but the waves reach it staggered and its cost hides)
 			int bpc = std::min(per_cu, 2);
-			if (const char* e = getenv("CALHIP_BATCH_COOP_BLOCKS_PER_CU"))
+			if (const char* e = getenv("CALHIP_BATCH_BLOCKS_PER_CU"))
 				if (atoi(e) > 0 && atoi(e) <= per_cu) bpc = atoi(e);
 			const int grid = bpc * cus;
 			const int chunk = per_launch > 0 ? per_launch : steps;
@@ -330,7 +331,10 @@ struct Runner {
 					return -1;
 			}
 		} else {
-			const int grid = 3 * cus;   // three blocks per CU (the sweep's occupancy at prefetch depth 1)
+			int bpc = 2;   // as the cooperative grid (771 ms either way; 3 per CU: 798)
+			if (const char* e = getenv("CALHIP_BATCH_BLOCKS_PER_CU"))
+				if (atoi(e) > 0 && atoi(e) <= 8) bpc = atoi(e);
+			const int grid = bpc * cus;
 			for (int t = 0; t < steps; ++t) {
 				if (depth >= 2)
 					hipLaunchKernelGGL(batch_step<2>, dim3(grid), dim3(BATCH_BLOCK), 0, stream, dinst, dti, rpw, z, h0,

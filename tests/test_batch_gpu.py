@@ -23,7 +23,7 @@ def test_batch_vs_oracle(steps, rpw, per_launch, bpc, monkeypatch):
     from opencal_amd.models import sciddicat_batch
     monkeypatch.setenv("CALHIP_BATCH_ROWS_PER_WAVE", rpw)
     monkeypatch.setenv("CALHIP_BATCH_STEPS_PER_LAUNCH", per_launch)
-    monkeypatch.setenv("CALHIP_BATCH_COOP_BLOCKS_PER_CU", bpc)
+    monkeypatch.setenv("CALHIP_BATCH_BLOCKS_PER_CU", bpc)
     specs = synth.batch_instances(12, seed=77, lo=24, hi=180)
     specs += [(6, 40, 0.5, 0.002, 901), (20, 6, 0.45, 0.001, 902), (9, 8, 0.6, 0.004, 903), (8, 130, 0.5, 0.001, 904)]
     insts = []
