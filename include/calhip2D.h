@@ -100,6 +100,22 @@ int calhipBindProcess2D(struct CALModel2D* ca2D, CALCallbackFunc2D process, cons
 int calhipCheckProcessSource2D(const char* source, const char* const* names, int n_names, char* log,
                                size_t log_bytes);
 
+/* 1 when the generic device program runs the process `device_name` of
+ * `source` in colour phases, 0 when it runs every cell at once, -1 on a bad
+ * argument (no GPU is needed; 2D and 3D sources alike).  A process runs in
+ * phases when it, or a function of the source it calls, uses an operation
+ * that reaches another cell's values: calSetX*, calSetCurrentX*, calInitX*,
+ * calAddNextX*, calGetNextX* (OpenCAL-OMP/source/cal2DUnsafe.c,
+ * cal3DUnsafe.c).  For a neighbourhood of radius r each axis is split into
+ * colour classes of period 2r + 1 and the process is launched once per class
+ * tuple, in row-major order of the tuples, so that no two cells that run
+ * together reach a common cell: the result is deterministic and race-free;
+ * it is the serial engine's visit reordered by phase, so sums that a cell
+ * receives from several neighbours may round differently from the serial
+ * row-major order.  No counterpart in the reference (its OpenMP engine
+ * locks each access instead). */
+int calhipProcessRunsInPhases(const char* source, const char* device_name);
+
 /* Mirror the host variable at host_address (bytes long) into the
  * file-scope variable `name` of the source before each transition.  Words that
  * hold the address of one of this model's substates, or of the model, are

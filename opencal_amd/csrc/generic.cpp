@@ -8,7 +8,9 @@
 // active cells, the unsafe operations) and is compiled with hipRTC together
 // with the device side of that API (generic_api.inc) and one kernel per
 // process that calls it for every cell of the grid, or every cell of the
-// active list.  Every file-scope variable of the source is also a device
+// active list; a process that writes its neighbours (the unsafe operations)
+// runs in colour phases, so that no two threads touch one cell (see
+// processes_touching_neighbours below).  Every file-scope variable of the source is also a device
 // variable; those the processes read are mirrored from the host
 // (calhipBindGlobal2D/3D) before each transition, and pointer-sized words
 // that hold the host address of one of the model's substates, or of the
@@ -24,12 +26,16 @@
 //
 // Compilation: one code object per distinct source text, cached for the life
 // of the process; each model loads its own module (its own globals).
+#include <ctype.h>
 #include <hip/hiprtc.h>
+#include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -342,32 +348,240 @@ std::string process_program_text(const std::string& source, const std::vector<st
 	     "#line 1 \"process source\"\n";
 	t += source;
 	t += "\n#pragma clang attribute pop\n#pragma clang force_cuda_host_device end\n#undef main\n";
+	// Every kernel takes a colour phase: period p (0: no phase, every cell)
+	// and the phase's class on each axis.  A phase of the whole grid
+	// enumerates its own cells; a phase of the active list visits the list's
+	// cells of that class.
 	for (size_t k = 0; k < names.size() && dims == 3; ++k) {
 		t += "extern \"C\" __global__ void __launch_bounds__(256) calhip_proc_" + std::to_string(k) +
-		     "(struct CALModel3D* m, const int* cells, const int* count)\n{\n"
+		     "(struct CALModel3D* m, const int* cells, const int* count, int ph_p, int ph_i, int ph_j, int ph_k)\n{\n"
 		     "\tconst int cnt = count ? count[0] : 0;\n"
 		     "\tconst unsigned long long cols = (unsigned long long)m->columns;\n"
 		     "\tconst unsigned long long plane = (unsigned long long)m->rows * cols;\n"
+		     "\tconst unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;\n"
+		     "\tconst unsigned long long t0 = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+		     "\tif (ph_p > 0 && cnt == 0) {\n"
+		     "\t\tconst unsigned long long ni = calhip_class_size(m->rows, ph_p, ph_i);\n"
+		     "\t\tconst unsigned long long nj = calhip_class_size(m->columns, ph_p, ph_j);\n"
+		     "\t\tconst unsigned long long nk = calhip_class_size(m->slices, ph_p, ph_k);\n"
+		     "\t\tfor (unsigned long long t = t0; t < ni * nj * nk; t += stride) {\n"
+		     "\t\t\tconst int i = calhip_class_at(m->rows, ph_p, ph_i, (int)(t / (nj * nk)));\n"
+		     "\t\t\tconst int j = calhip_class_at(m->columns, ph_p, ph_j, (int)(t / nk % nj));\n"
+		     "\t\t\tconst int kk = calhip_class_at(m->slices, ph_p, ph_k, (int)(t % nk));\n"
+		     "\t\t\t" + names[k] + "(m, i, j, kk);\n\t\t}\n\t\treturn;\n\t}\n"
 		     "\tconst unsigned long long total = cnt > 0 ? (unsigned long long)cnt : plane * m->slices;\n"
-		     "\tfor (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;"
-		     " t += (unsigned long long)gridDim.x * blockDim.x) {\n"
+		     "\tfor (unsigned long long t = t0; t < total; t += stride) {\n"
 		     "\t\tconst unsigned long long c = cnt > 0 ? (unsigned long long)cells[t] : t;\n"
 		     "\t\tconst unsigned long long r = c % plane;\n"
-		     "\t\t" + names[k] + "(m, (int)(r / cols), (int)(r % cols), (int)(c / plane));\n\t}\n}\n";
+		     "\t\tconst int i = (int)(r / cols), j = (int)(r % cols), kk = (int)(c / plane);\n"
+		     "\t\tif (ph_p > 0 && (calhip_class(i, m->rows, ph_p) != ph_i || calhip_class(j, m->columns, ph_p) != ph_j ||"
+		     " calhip_class(kk, m->slices, ph_p) != ph_k))\n\t\t\tcontinue;\n"
+		     "\t\t" + names[k] + "(m, i, j, kk);\n\t}\n}\n";
 	}
 	for (size_t k = 0; k < names.size() && dims != 3; ++k) {
 		t += "extern \"C\" __global__ void __launch_bounds__(256) calhip_proc_" + std::to_string(k) +
-		     "(struct CALModel2D* m, const int* cells, const int* count)\n{\n"
+		     "(struct CALModel2D* m, const int* cells, const int* count, int ph_p, int ph_i, int ph_j, int)\n{\n"
 		     "\tconst int cnt = count ? count[0] : 0;\n"
 		     "\tconst int cols = m->columns;\n"
+		     "\tconst unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;\n"
+		     "\tconst unsigned long long t0 = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
+		     "\tif (ph_p > 0 && cnt == 0) {\n"
+		     "\t\tconst unsigned long long ni = calhip_class_size(m->rows, ph_p, ph_i);\n"
+		     "\t\tconst unsigned long long nj = calhip_class_size(cols, ph_p, ph_j);\n"
+		     "\t\tfor (unsigned long long t = t0; t < ni * nj; t += stride)\n"
+		     "\t\t\t" + names[k] + "(m, calhip_class_at(m->rows, ph_p, ph_i, (int)(t / nj)),"
+		     " calhip_class_at(cols, ph_p, ph_j, (int)(t % nj)));\n\t\treturn;\n\t}\n"
 		     "\tconst unsigned long long total = cnt > 0 ? (unsigned long long)cnt : "
 		     "(unsigned long long)m->rows * cols;\n"
-		     "\tfor (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;"
-		     " t += (unsigned long long)gridDim.x * blockDim.x) {\n"
+		     "\tfor (unsigned long long t = t0; t < total; t += stride) {\n"
 		     "\t\tconst unsigned long long c = cnt > 0 ? (unsigned long long)cells[t] : t;\n"
-		     "\t\t" + names[k] + "(m, (int)(c / cols), (int)(c % cols));\n\t}\n}\n";
+		     "\t\tconst int i = (int)(c / cols), j = (int)(c % cols);\n"
+		     "\t\tif (ph_p > 0 && (calhip_class(i, m->rows, ph_p) != ph_i || calhip_class(j, cols, ph_p) != ph_j))"
+		     " continue;\n"
+		     "\t\t" + names[k] + "(m, i, j);\n\t}\n}\n";
 	}
 	return t;
+}
+
+// ---- processes that touch other cells ---------------------------------------
+//
+// The unsafe operations (OpenCAL-OMP/source/cal2DUnsafe.c, cal3DUnsafe.c)
+// let a cell write its neighbours' next or current values and read their
+// next values.  Run one thread per cell, such a process would race with its
+// neighbours' threads (the reference's OpenMP engine takes a lock per access,
+// which keeps each access whole but leaves the order, and a
+// read-modify-write written as a get and a set, to the scheduler).  A process
+// that reaches one of these operations therefore runs in colour phases: for
+// a neighbourhood of radius r, colour period p = 2r + 1 on every axis (the
+// classes of calhip_class), one launch per class tuple in row-major order of
+// the tuples.  Two cells of one phase lie p or more apart on some axis, so
+// the cells they reach (radius r each) are disjoint: within a phase no two
+// threads touch the same cell, and the result does not depend on the
+// scheduling.  It is the serial engine's result with the visits reordered
+// by phase; the sums a cell receives from several neighbours may round
+// differently from the serial row-major order, by the last bit.
+//
+// The analysis is textual: the process's function and every function of the
+// source it calls, transitively, are searched for calls of the unsafe
+// operations; a use of them outside every function body (a macro) marks
+// every process.
+
+bool ident_char(char c) { return (c >= 'a' && c <= 'z') || (c >= 'A' && c <= 'Z') || (c >= '0' && c <= '9') || c == '_'; }
+
+// The source with comments and string and character literals blanked out
+// (newlines kept).
+std::string strip_comments_and_literals(const std::string& s)
+{
+	std::string o = s;
+	const size_t n = s.size();
+	size_t i = 0;
+	while (i < n) {
+		if (s[i] == '/' && i + 1 < n && s[i + 1] == '/') {
+			while (i < n && s[i] != '\n') o[i++] = ' ';
+		} else if (s[i] == '/' && i + 1 < n && s[i + 1] == '*') {
+			o[i] = o[i + 1] = ' ';
+			i += 2;
+			while (i < n && !(s[i] == '*' && i + 1 < n && s[i + 1] == '/')) {
+				if (s[i] != '\n') o[i] = ' ';
+				++i;
+			}
+			if (i < n) {
+				o[i] = o[i + 1] = ' ';
+				i += 2;
+			}
+		} else if (s[i] == '"' || s[i] == '\'') {
+			const char q = s[i];
+			o[i++] = ' ';
+			while (i < n && s[i] != q && s[i] != '\n') {
+				if (s[i] == '\\' && i + 1 < n) o[i++] = ' ';
+				if (s[i] != '\n') o[i] = ' ';
+				++i;
+			}
+			if (i < n && s[i] == q) o[i++] = ' ';
+		} else {
+			++i;
+		}
+	}
+	return o;
+}
+
+struct Body {
+	size_t begin, end;   // the braces
+};
+
+// File-scope function definitions: an identifier, a parenthesised list and
+// a brace at depth 0.
+std::map<std::string, Body> function_bodies(const std::string& s)
+{
+	std::map<std::string, Body> out;
+	int depth = 0;
+	std::string pending;
+	size_t start = 0;
+	for (size_t i = 0; i < s.size(); ++i) {
+		if (s[i] == '{') {
+			if (depth == 0) {
+				pending.clear();
+				size_t k = i;
+				while (k > 0 && isspace((unsigned char)s[k - 1])) --k;
+				if (k > 0 && s[k - 1] == ')') {
+					int pd = 0;
+					size_t q = k - 1;
+					for (;;) {
+						if (s[q] == ')') ++pd;
+						if (s[q] == '(' && --pd == 0) break;
+						if (q == 0) break;
+						--q;
+					}
+					if (pd == 0) {
+						size_t e = q;
+						while (e > 0 && isspace((unsigned char)s[e - 1])) --e;
+						size_t b = e;
+						while (b > 0 && ident_char(s[b - 1])) --b;
+						if (b < e) pending = s.substr(b, e - b);
+					}
+				}
+				start = i;
+			}
+			++depth;
+		} else if (s[i] == '}' && depth > 0) {
+			if (--depth == 0 && !pending.empty()) {
+				out[pending] = {start, i};
+				pending.clear();
+			}
+		}
+	}
+	return out;
+}
+
+// calSetX, calSetCurrentX, calInitX, calAddNextX, calGetNextX of 2D or 3D
+// substates of any type.
+bool unsafe_operation(const std::string& id)
+{
+	static const char* kPrefix[] = {"calSetX", "calSetCurrentX", "calInitX", "calAddNextX", "calGetNextX"};
+	for (const char* p : kPrefix) {
+		const size_t n = strlen(p);
+		if (id.size() == n + 3 && id.compare(0, n, p) == 0 && (id[n] == '2' || id[n] == '3') && id[n + 1] == 'D' &&
+		    (id[n + 2] == 'b' || id[n + 2] == 'i' || id[n + 2] == 'r'))
+			return true;
+	}
+	return false;
+}
+
+template <class F>
+void for_each_identifier(const std::string& s, size_t begin, size_t end, F f)
+{
+	size_t i = begin;
+	while (i < end) {
+		if (ident_char(s[i]) && !(s[i] >= '0' && s[i] <= '9') && (i == 0 || !ident_char(s[i - 1]))) {
+			size_t e = i;
+			while (e < end && ident_char(s[e])) ++e;
+			f(s.substr(i, e - i));
+			i = e;
+		} else {
+			++i;
+		}
+	}
+}
+
+// Per process name: true when it reaches an unsafe operation.
+std::vector<char> processes_touching_neighbours(const std::string& source, const std::vector<std::string>& names)
+{
+	const std::string s = strip_comments_and_literals(source);
+	const std::map<std::string, Body> bodies = function_bodies(s);
+	// a use outside every body (a macro) marks every process
+	bool outside = false, any = false;
+	size_t at = 0;
+	for_each_identifier(s, 0, s.size(), [&](const std::string& id) { any = any || unsafe_operation(id); });
+	std::vector<Body> sorted;
+	for (const auto& b : bodies) sorted.push_back(b.second);
+	std::sort(sorted.begin(), sorted.end(), [](const Body& x, const Body& y) { return x.begin < y.begin; });
+	for (const Body& b : sorted) {
+		for_each_identifier(s, at, b.begin, [&](const std::string& id) { outside = outside || unsafe_operation(id); });
+		at = b.end + 1;
+	}
+	for_each_identifier(s, at, s.size(), [&](const std::string& id) { outside = outside || unsafe_operation(id); });
+	std::vector<char> out(names.size(), outside ? 1 : 0);
+	for (size_t k = 0; k < names.size() && !outside; ++k) {
+		std::vector<std::string> todo{names[k]};
+		std::set<std::string> seen{names[k]};
+		// a process not found as a plain definition may reach any use
+		if (bodies.find(names[k]) == bodies.end()) {
+			out[k] = any ? 1 : 0;
+			continue;
+		}
+		bool hit = false;
+		while (!todo.empty() && !hit) {
+			const Body b = bodies.at(todo.back());
+			todo.pop_back();
+			for_each_identifier(s, b.begin, b.end, [&](const std::string& id) {
+				if (unsafe_operation(id)) hit = true;
+				else if (bodies.count(id) && seen.insert(id).second)
+					todo.push_back(id);
+			});
+		}
+		out[k] = hit ? 1 : 0;
+	}
+	return out;
 }
 
 int grid_for(size_t n)
@@ -408,6 +622,8 @@ struct GenericProgram : Program {
 
 	hipModule_t module = nullptr;
 	std::vector<hipFunction_t> fns;   // per entry of procs
+	std::vector<char> phased;         // per entry of procs: runs in colour phases
+	int period = 1;                   // colour period: 2 x the neighbourhood's radius + 1
 	std::string built_text;
 	void* d_view = nullptr;
 	void* d_X = nullptr;
@@ -449,6 +665,11 @@ struct GenericProgram : Program {
 			if (CALHIP_CHECK(hipModuleGetFunction(&f, module, kn.c_str()))) return -1;
 			fns.push_back(f);
 		}
+		{
+			std::vector<std::string> names;
+			for (const Bound& b : procs) names.push_back(b.name);
+			phased = processes_touching_neighbours(source, names);
+		}
 		for (Global& g : globals) {
 			hipDeviceptr_t p = nullptr;
 			size_t bytes = 0;
@@ -478,6 +699,9 @@ struct GenericProgram : Program {
 		if (CALHIP_CHECK(hipMalloc(&d_X, sizeof(CALCell2D) * (size_t)(nx > 0 ? nx : 1)))) return -1;
 		if (nx > 0 && CALHIP_CHECK(hipMemcpy(d_X, p->X, sizeof(CALCell2D) * (size_t)nx, hipMemcpyHostToDevice)))
 			return -1;
+		int radius = 0;
+		for (int n = 0; n < nx; ++n) radius = std::max(radius, std::max(std::abs(p->X[n].i), std::abs(p->X[n].j)));
+		period = 2 * radius + 1;
 		DevModelView v = {};
 		v.rows = p->rows;
 		v.columns = p->columns;
@@ -504,6 +728,10 @@ struct GenericProgram : Program {
 		if (CALHIP_CHECK(hipMalloc(&d_X, sizeof(CALCell3D) * (size_t)(nx > 0 ? nx : 1)))) return -1;
 		if (nx > 0 && CALHIP_CHECK(hipMemcpy(d_X, p->X, sizeof(CALCell3D) * (size_t)nx, hipMemcpyHostToDevice)))
 			return -1;
+		int radius = 0;
+		for (int n = 0; n < nx; ++n)
+			radius = std::max({radius, std::abs(p->X[n].i), std::abs(p->X[n].j), std::abs(p->X[n].k)});
+		period = 2 * radius + 1;
 		DevModelView3 v = {};
 		v.rows = p->rows;
 		v.columns = p->columns;
@@ -584,20 +812,45 @@ struct GenericProgram : Program {
 		return -1;
 	}
 
+	// Cells of colour class c on an axis of n cells (the device's
+	// calhip_class_size).
+	static long long class_size(int n, int p, int c) { return c < p ? n / p : 1; }
+
 	int launch(ModelX* m, int k)
 	{
 		if (m->active() && device_rebuild_active(m)) return -1;
 		const int* cells = m->active() ? m->d_cells : nullptr;
 		const int* count = m->active() ? m->d_count : nullptr;
 		void* view = d_view;
-		void* args[] = {&view, &cells, &count};
+		int ph[4] = {0, 0, 0, 0};   // period, class of rows, columns, slices
+		void* args[] = {&view, &cells, &count, &ph[0], &ph[1], &ph[2], &ph[3]};
+		const int rows = m->pub3 ? m->pub3->rows : m->pub.rows;
+		const int cols = m->pub3 ? m->pub3->columns : m->pub.columns;
+		const int slices = m->pub3 ? m->pub3->slices : 1;
+		const int p = phased[(size_t)k] ? period : 0;
+		// colour classes per axis (one class and no phase for an unphased process)
+		const int ni = p ? p + rows % p : 1, nj = p ? p + cols % p : 1, nk = p && m->pub3 ? p + slices % p : 1;
 		hipEvent_t ev;
 		timer_begin(m, &ev);
-		if (CALHIP_CHECK(hipModuleLaunchKernel(fns[(size_t)k], (unsigned)grid_for(m->cells()), 1, 1, 256, 1, 1, 0,
-		                                       m->stream, args, nullptr)))
-			return -1;
+		for (int ci = 0; ci < ni; ++ci)
+			for (int cj = 0; cj < nj; ++cj)
+				for (int ck = 0; ck < nk; ++ck) {
+					size_t n = m->cells();
+					if (p && !m->active()) {
+						n = (size_t)(class_size(rows, p, ci) * class_size(cols, p, cj) *
+						             (m->pub3 ? class_size(slices, p, ck) : 1));
+						if (n == 0) continue;
+					}
+					ph[0] = p;
+					ph[1] = ci;
+					ph[2] = cj;
+					ph[3] = ck;
+					if (CALHIP_CHECK(hipModuleLaunchKernel(fns[(size_t)k], (unsigned)grid_for(n), 1, 1, 256, 1, 1, 0,
+					                                       m->stream, args, nullptr)))
+						return -1;
+					m->stats.launches++;
+				}
 		timer_end(m, ev);
-		m->stats.launches++;
 		if (m->active()) m->flags_dirty = true;   // the process may have added or removed cells
 		return 0;
 	}
@@ -812,4 +1065,13 @@ extern "C" int calhipCheckProcessSource3D(const char* source, const char* const*
                                           size_t log_bytes)
 {
 	return check_source(source, names, n_names, log, log_bytes, 3, "calhipCheckProcessSource3D");
+}
+
+extern "C" int calhipProcessRunsInPhases(const char* source, const char* device_name)
+{
+	if (!source || !valid_identifier(device_name)) {
+		set_error(64, "calhipProcessRunsInPhases: give the source and the C name of the process");
+		return -1;
+	}
+	return processes_touching_neighbours(source, {device_name})[0] ? 1 : 0;
 }

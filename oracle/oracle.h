@@ -26,6 +26,13 @@ int orc_sciddicat_run(int rows, int cols, const double* z, const double* h,
                       double p_r, double p_eps, int steps, int active,
                       double* out_z, double* out_h, int* active_counts);
 
+/* The unsafe SciddicaT of cal_sciddicaT-unsafe-glut on the serial engine
+ * (active cells, next-layer read-modify-writes to the neighbours).  order 0:
+ * the serial visit order; 1: the device's colour phases.  z, h before the
+ * init (which lowers z where h > 0); out_active: the final list length. */
+int orc_sciddicat_unsafe_run(int rows, int cols, const double* z0, const double* h0, double p_r, double p_eps,
+                             int steps, int order, double* out_h, int* out_active);
+
 /* Conway Life, Moore, toroidal, uint8 0/1 cells. */
 int orc_life_run(int rows, int cols, const unsigned char* alive0, int steps,
                  unsigned char* out);

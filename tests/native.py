@@ -162,3 +162,19 @@ def ref_sciara(inp, steps, active=True):
     if lib is None:
         return None
     return _sciara_call(lib.ref_sciara_run, inp, steps, active, ())
+
+
+def oracle_sciddicat_unsafe(z, h, p_r, p_eps, steps, order=0):
+    """The unsafe SciddicaT (next-layer read-modify-writes to the
+    neighbours, active cells) of cal_sciddicaT-unsafe-glut; order 0 visits
+    the cells serially, 1 in the device's colour phases.  Returns (h, final
+    active-list length)."""
+    z = np.ascontiguousarray(z, np.float64)
+    h = np.ascontiguousarray(h, np.float64)
+    rows, cols = z.shape
+    oh = np.empty_like(h)
+    na = C.c_int(0)
+    rc = oracle().orc_sciddicat_unsafe_run(rows, cols, dptr(z), dptr(h), C.c_double(p_r),
+                                           C.c_double(p_eps), steps, order, dptr(oh), C.byref(na))
+    assert rc == 0
+    return oh, na.value

@@ -180,3 +180,71 @@ def test_reference_mbusu_3d_state_against_the_reference_engine(tmp_path):
         exact = int(np.sum(g[s] == c[s]))
         print(f"mbusu {name}: max |diff| / max |value| {rel:.2e}, {exact}/{g.shape[1]} bit-identical", flush=True)
         assert rel <= 1e-12, name
+
+
+BIN_U = os.path.join(native.REF_DIR, "dropin_sciddicaT_unsafe")
+
+
+def _unsafe_data(tmp_path, z=None, h=None):
+    data = tmp_path / "data"
+    data.mkdir()
+    if z is None:
+        for name, src in (("dem.txt", "sciddicaT_610x496_dem.txt.gz"),
+                          ("source.txt", "sciddicaT_610x496_source.txt.gz")):
+            with gzip.open(os.path.join(GOLDEN, src), "rb") as f, open(data / name, "wb") as g:
+                shutil.copyfileobj(f, g)
+    else:
+        import numpy as np
+        np.savetxt(data / "dem.txt", z, fmt="%.17g")
+        np.savetxt(data / "source.txt", h, fmt="%.17g")
+    return data
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_U), reason="dropin_sciddicaT_unsafe not built (needs the reference tree)")
+def test_reference_unsafe_program_on_the_generic_device_program(tmp_path):
+    """The reference example cal_sciddicaT-unsafe-glut (its model file
+    unchanged: the transition writes its outflows into the neighbours' next
+    layer with calSetX2Dr / calGetNextX2Dr and marks them active), its two
+    processes compiled for the GPU at run time; the transition runs in colour
+    phases.  On the example's data: the raw h equals the oracle in both visit
+    orders bit for bit, the saved file equals what the reference OpenCAL-OMP
+    engine writes for this program ("%.9f"), and printed with the serial
+    engine's "%f" it is the example's shipped width_final.txt."""
+    import numpy as np
+    from opencal_amd import synth
+    print("case dropin sciddicaT unsafe 610x496 4000 steps", flush=True)
+    data = _unsafe_data(tmp_path)
+    r = subprocess.run([BIN_U, "h.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "steps 4000 active 5507" in r.stdout, r.stdout + r.stderr
+    want9 = gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
+    assert (data / "width_final.txt").read_text() == want9
+    got = np.fromfile(tmp_path / "h.bin", np.float64).reshape(610, 496)
+    z = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_dem.txt.gz"))
+    h = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_source.txt.gz"))
+    for order in (1, 0):
+        o, n_active = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 4000, order)
+        assert np.array_equal(got, o) and n_active == 5507, order
+    shipped = gzip.open(os.path.join(GOLDEN, "sciddicaT_active_610x496_width_final.txt.gz"), "rt").read()
+    assert "".join("".join("%f  " % v for v in row) + "\n" for row in got) == shipped
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_U), reason="dropin_sciddicaT_unsafe not built (needs the reference tree)")
+def test_reference_unsafe_program_on_synthetic_data(tmp_path):
+    """The same program on a synthetic slope and source: bit for bit the
+    oracle's colour-phase visit; within 1e-12 of the serial visit (the
+    reference engine's, which the oracle's serial order equals bit for bit),
+    with the same mass."""
+    import numpy as np
+    from opencal_amd import synth
+    print("case dropin sciddicaT unsafe synthetic 610x496 seed 5", flush=True)
+    z, h, _ = synth.sciddicat_inputs(610, 496, seed=5, patch=10)
+    _unsafe_data(tmp_path, z, h)
+    r = subprocess.run([BIN_U, "h.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(tmp_path / "h.bin", np.float64).reshape(610, 496)
+    phased, _ = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 4000, 1)
+    serial, _ = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 4000, 0)
+    assert np.array_equal(got, phased)
+    assert np.abs(got - serial).max() <= 1e-12 * h.max()
+    assert abs(got.sum() - h.sum()) <= 1e-12 * h.sum()

@@ -43,6 +43,50 @@ def test_bad_source_reports_the_compiler():
     L.calhipClearError()
 
 
+def test_processes_that_reach_other_cells_run_in_phases():
+    """The generic program runs a process in colour phases when it, or a
+    function of the source it calls, uses an operation that reaches another
+    cell (calSetX*, calSetCurrentX*, calInitX*, calAddNextX*, calGetNextX*);
+    mentions in comments and strings do not count; a use outside every
+    function (a macro) marks every process."""
+    L = lib()
+    L.calhipProcessRunsInPhases.argtypes = [C.c_char_p, C.c_char_p]
+    src = b"""
+    #include <cal2DUnsafe.h>
+    /* calSetX2Dr in a comment */
+    static void give(struct CALModel2D* ca, int i, int j, int n, double f)
+    {
+        calSetX2Dr(ca, Q.h, i, j, n, calGetNextX2Dr(ca, Q.h, i, j, n) + f);
+    }
+    static double half(double x) { return 0.5 * x; }
+    void flows(struct CALModel2D* ca, int i, int j)
+    {
+        int n;
+        for (n = 1; n < ca->sizeof_X; n++) give(ca, i, j, n, half(calGet2Dr(ca, Q.h, i, j)));
+    }
+    void mark(struct CALModel2D* ca, int i, int j)
+    {
+        // calAddNextX2Dr(ca, Q.h, i, j, 1, 1.0);
+        const char* s = "calInitX2Dr";
+        if (calGet2Dr(ca, Q.h, i, j) > 0) calAddActiveCellX2D(ca, i, j, 1);
+    }
+    void self_only(struct CALModel2D* ca, int i, int j) { calSet2Dr(ca, Q.h, i, j, half(calGetNext2Dr(ca, Q.h, i, j))); }
+    void gift3(struct CALModel3D* ca, int i, int j, int k) { calAddNextX3Di(ca, Q.q, i, j, k, 1, 1); }
+    """
+    assert L.calhipProcessRunsInPhases(src, b"flows") == 1
+    assert L.calhipProcessRunsInPhases(src, b"mark") == 0
+    assert L.calhipProcessRunsInPhases(src, b"self_only") == 0
+    assert L.calhipProcessRunsInPhases(src, b"gift3") == 1
+    # a name the source does not define plainly: it may reach any use
+    assert L.calhipProcessRunsInPhases(src, b"elsewhere") == 1
+    assert L.calhipProcessRunsInPhases(b"void a(struct CALModel2D* c, int i, int j) {}", b"elsewhere") == 0
+    macro = src + b"#define GIVE(ca, i, j, n, f) calSetCurrentX2Dr(ca, Q.h, i, j, n, f)\n"
+    assert L.calhipProcessRunsInPhases(macro, b"self_only") == 1
+    L.calhipClearError()
+    assert L.calhipProcessRunsInPhases(src, b"not a name") == -1
+    L.calhipClearError()
+
+
 # ------------------------------------------------------------------- GPU
 
 def _gsct(z, h, p_r, p_eps, steps, active):

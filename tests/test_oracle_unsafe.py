@@ -1,0 +1,116 @@
+"""The oracle of the unsafe SciddicaT (oracle/oracle_sciddicat_unsafe.c: the
+model of OpenCAL/examples/cal_sciddicaT-unsafe-glut, whose transition
+writes its outflows into the neighbours' next layer, on the serial engine
+with active cells), pinned by:
+
+- the example's shipped output, cal_sciddicaT-unsafe-glut/data/width_final.txt
+  (610 x 496, 4000 steps, written by the serial engine with "%f "); that file
+  is byte-identical to tests/golden/sciddicaT_active_610x496_width_final.txt.gz
+  (the active-cells example's shipped output), which stands for it here;
+- the same program built on the reference OpenCAL-OMP engine
+  (oracle/_ref/calomp_sciddicaT_unsafe, one thread: the serial row-major
+  visit), bit for bit on the example's data and on a synthetic one;
+- mass conservation on the torus (every outflow leaves one cell and enters
+  another).
+
+order 1 restates the device's colour-phased visit; it differs from the
+serial visit only in the order in which a cell's next value receives its
+neighbours' outflows, i.e. by rounding."""
+import functools
+import gzip
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import native
+from opencal_amd import synth
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REF_BIN = os.path.join(native.REF_DIR, "calomp_sciddicaT_unsafe")
+
+
+def _example_inputs():
+    z = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_dem.txt.gz"))
+    h = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_source.txt.gz"))
+    return z, h
+
+
+@functools.lru_cache(maxsize=None)
+def _example_run(order):
+    z, h = _example_inputs()
+    return native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 4000, order)
+
+
+def _format(h, fmt):
+    """A matrix as the engines' calSaveSubstate2Dr writes it: each value
+    printed with fmt and a space, then a space; a newline per row."""
+    return "".join("".join(fmt % v + "  " for v in row) + "\n" for row in h)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_oracle_reproduces_the_shipped_output(order):
+    _, h = _example_inputs()
+    out, n_active = _example_run(order)
+    # the serial engine's "%f" file, shipped with the example
+    want = gzip.open(os.path.join(GOLDEN, "sciddicaT_active_610x496_width_final.txt.gz"), "rt").read()
+    assert _format(out, "%f") == want
+    # the OpenMP engine's "%.9f" file of the same program (what
+    # calomp_sciddicaT_unsafe writes; equal to the full-grid example's output)
+    want9 = gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
+    assert _format(out, "%.9f") == want9
+    assert n_active == 5507
+    # mass conservation: the source's volume, to rounding
+    assert abs(out.sum() - h.sum()) <= 1e-9 * h.sum()
+
+
+def test_both_orders_agree_bitwise_on_the_example():
+    (a, na), (b, nb) = _example_run(0), _example_run(1)
+    assert np.array_equal(a, b) and na == nb
+
+
+@pytest.mark.parametrize("shape,seed", [((64, 64), 1), ((67, 71), 2), ((31, 29), 4), ((5, 7), 6)])
+def test_phase_order_differs_from_serial_only_by_rounding(shape, seed):
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(0, 50, shape).cumsum(0) * 0.1
+    h = np.where(rng.random(shape) < 0.2, rng.uniform(0, 5, shape), 0.0)
+    a, _ = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 300, 0)
+    b, _ = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 300, 1)
+    assert np.abs(a - b).max() <= 1e-12 * max(1.0, h.max())
+    for out in (a, b):
+        assert abs(out.sum() - h.sum()) <= 1e-12 * h.sum()
+        assert out.min() > -1e-12
+
+
+def _run_reference(tmp_path, z, h):
+    data = tmp_path / "data"
+    data.mkdir()
+    np.savetxt(data / "dem.txt", z, fmt="%.17g")
+    np.savetxt(data / "source.txt", h, fmt="%.17g")
+    r = subprocess.run([REF_BIN, "h.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    return np.fromfile(tmp_path / "h.bin", np.float64).reshape(z.shape), r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="calomp_sciddicaT_unsafe not built (needs the reference tree)")
+def test_reference_engine_on_the_example_matches_the_serial_oracle(tmp_path):
+    z, h = _example_inputs()
+    ref, out = _run_reference(tmp_path, z, h)
+    assert "steps 4000 active 5507" in out
+    assert np.array_equal(_example_run(0)[0], ref)
+    want9 = gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
+    assert (tmp_path / "data" / "width_final.txt").read_text() == want9
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="calomp_sciddicaT_unsafe not built (needs the reference tree)")
+def test_reference_engine_on_synthetic_data_matches_the_serial_oracle(tmp_path):
+    """A synthetic slope and source (the program's fixed 610 x 496 and 4000
+    steps): the serial oracle bit for bit; the phase order within 1e-12."""
+    z, h, _ = synth.sciddicat_inputs(610, 496, seed=5, patch=10)
+    ref, _ = _run_reference(tmp_path, z, h)
+    o, _ = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 4000, 0)
+    assert np.array_equal(o, ref)
+    p, _ = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 4000, 1)
+    assert np.abs(p - ref).max() <= 1e-12 * h.max()
