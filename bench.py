@@ -72,11 +72,11 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024, batched=True):
     from opencal_amd import synth
     from opencal_amd.models import SciddicaT
     z, h, _ = synth.sciddicat_inputs(n, n, seed)
-    m = SciddicaT(z, h, 0.5, 0.001, steps + warmup + 2, active=False)
+    m = SciddicaT(z, h, 0.5, 0.001, steps + warmup + 4, active=False)
     m.init()
     m.steps(warmup)          # the first step uploads the model
     if batched:
-        m.run_steps(2)       # chooses the two-step sweep's chunk length
+        m.run_steps(4)       # the two-step sweep's tile shape is chosen on first use
     m.sync()
     m.timing(True)
     t0 = time.perf_counter()

@@ -463,6 +463,8 @@ extern "C" void calGlobalTransitionFunction2D(struct CALModel2D* m)
 }
 
 namespace calhip {
+bool program_covers_model(ModelX* x) { return program_covers(x); }
+
 int device_transition(ModelX* x, bool with_steering)
 {
 	if (!program_covers(x)) {

@@ -101,7 +101,11 @@ extern "C" int calhipRunSteps2D(struct CALRun2D* r, int n)
 	                       (!r->steering || (twin && r->UPDATE_MODE == CAL_UPDATE_IMPLICIT));
 	int done = 0;
 	while (done < n) {
-		if (batchable && done > 0 && x->phase == Phase::Device) {
+		// The steps go to the program together once the model is on the
+		// device: from the first step when it already is (and the program
+		// covers its processes, which calRunCAStep2D would check), otherwise
+		// after the first step, which uploads it.
+		if (batchable && x->phase == Phase::Device && (done > 0 || program_covers_model(x))) {
 			const int rest = n - done;
 			device_changed(x);
 			if (x->prog->run_steps(x, rest, twin)) return -1;

@@ -209,6 +209,8 @@ int device_update_sub(ModelX* m, SubX* s);  // calUpdateSubstate of one written 
 int device_fill(ModelX* m, SubX* s, double v, bool both);
 int device_update(ModelX* m);
 int device_rebuild_active(ModelX* m);
+// The bound program runs every elementary process of the 2D model, in order.
+bool program_covers_model(ModelX* m);
 void flip(SubX* s);
 
 // calGlobalTransitionFunction2D on the device, optionally followed by the

@@ -585,6 +585,58 @@ __device__ __forceinline__ void wet_first_record(const WetSets& ws, int L, bool 
 	}
 }
 
+// Dry regions.  When every cell within S rows and S columns of the output
+// rows [r0, r1) of a strip has h <= eps, no cell that can reach an output
+// cell in S steps has an outflow: the flows process returns before reading z
+// for a dry cell, every flow a cell of the region exchanges is +0.0, and
+// each of an output cell's width terms adds (0.0 - 0.0): after S steps
+// h = h0 + 0.0, which turns -0.0 into +0.0 and leaves every other value as
+// it is.  dry_copy streams h over the region with a halo of two rows and two
+// columns (enough for S = 1 and S = 2; DRY_ROWS rows at a time, every load
+// of a batch in flight together), writes h + 0.0 to the output rows as it
+// goes and returns (wave-uniform) whether the region was dry; at the first
+// batch that holds a wet cell it stops and returns false, and the caller
+// runs the full sweep, which rewrites those rows.  z is not read.  Needs an
+// even column count.
+constexpr int DRY_ROWS = 16;
+
+__device__ __forceinline__ bool dry_copy(const double* __restrict__ h, double* __restrict__ hn, int rows, int cols,
+                                         int pitch, int strip, int r0, int r1, double eps)
+{
+	const int lane = threadIdx.x & 63;
+	const int j0 = strip * STRIP_AL;
+	const int ncol = min(STRIP_AL, cols - j0);   // even
+	const bool own = 2 * lane < ncol;
+	const int j = own ? j0 + 2 * lane : j0;
+	const int jR = j0 + ncol == cols ? 0 : j0 + ncol;
+	const int jL2 = j0 - 2 < 0 ? j0 - 2 + cols : j0 - 2;
+	// lane 0 also checks the pair (j0-2, j0-1), lane 1 the pair (jR, jR+1)
+	const int jh = lane == 0 ? jL2 : jR;
+	const bool h_on = lane < 2;
+	for (int base = r0 - 2; base < r1 + 2; base += DRY_ROWS) {
+		double2 v[DRY_ROWS], x[DRY_ROWS];
+#pragma unroll
+		for (int k = 0; k < DRY_ROWS; ++k) {
+			int rr = min(base + k, r1 + 1) % rows;   // wave-uniform; any row count
+			rr = rr < 0 ? rr + rows : rr;
+			const size_t o = (size_t)rr * (size_t)pitch;
+			v[k] = *reinterpret_cast<const double2*>(h + o + j);
+			x[k] = h_on ? *reinterpret_cast<const double2*>(h + o + jh) : make_double2(0.0, 0.0);
+		}
+		bool wet = false;
+#pragma unroll
+		for (int k = 0; k < DRY_ROWS; ++k) {
+			wet = wet || !(v[k].x <= eps) || !(v[k].y <= eps) || (h_on && (!(x[k].x <= eps) || !(x[k].y <= eps)));
+			const int rr = base + k;
+			if (own && rr >= r0 && rr < r1)
+				*reinterpret_cast<double2*>(hn + (size_t)rr * (size_t)pitch + j) =
+				    make_double2(v[k].x + 0.0, v[k].y + 0.0);
+		}
+		if (__any((int)wet)) return false;
+	}
+	return true;
+}
+
 // ws.f_out == nullptr: plain grid order (gridDim.x = 8 * per_xcd); otherwise
 // wet-first order (gridDim.x = 8 * (per_xcd + cap)).
 template <int D>
@@ -601,7 +653,10 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) __attribute__((amdgpu_waves_per_e
 	const int r0 = chunk * rpw;
 	bool wet = false;
 	if (r0 < rows)
-		wet = sweep_chunk_al<D>(z, h, hn, trash, rows, cols, cols, strip, chunk, r0, min(r0 + rpw, rows), r, eps);
+		wet = dry_copy(h, hn, rows, cols, cols, strip, r0, min(r0 + rpw, rows), eps)
+		          ? false
+		          : sweep_chunk_al<D>(z, h, hn, trash, rows, cols, cols, strip, chunk, r0, min(r0 + rpw, rows), r,
+		                              eps);
 	wet_first_record(ws, L, wet);
 }
 

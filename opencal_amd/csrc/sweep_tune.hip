@@ -112,8 +112,11 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) __attribute__((amdgpu_waves_per_e
 			int e = c + 1;
 			while (e < cs1 && !((listed >> (e - cs0)) & 1u)) ++e;
 			const int r0 = c * g.s_rows, r1 = min(e * g.s_rows, rows);
-			const bool wet = (c & 1) ? sweep2_rows<true>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps)
-			                         : sweep2_rows<false>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps);
+			// a run that is dry within two cells is h + 0.0 (dry_copy)
+			const bool wet = dry_copy(h, hn, rows, cols, cols, strip, r0, r1, eps)
+			                     ? false
+			                     : ((c & 1) ? sweep2_rows<true>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps)
+			                                : sweep2_rows<false>(z, h, hn, trash, rows, cols, strip, r0, r1, r, eps));
 			if (wet) my_wet |= ((1u << (e - c)) - 1u) << (c - cs0);
 			c = e;
 		}
