@@ -12,8 +12,9 @@ beside it (`per_step_driver`).  Unit: giga cell updates per second.
 
 The same JSON line carries
   roofline      the fused sweep kernel's achieved algorithmic bandwidth
-                (24 B per cell update: z read, h read, h written) from HIP
-                events on the model's stream, against the 8 TB/s HBM peak;
+                (16 B per cell and launch: h read, h written; z is read only
+                near wet cells) from HIP events on the model's stream, against
+                the 8 TB/s HBM peak, and its measured HBM traffic;
   cpu_baseline  the reference OpenCAL-OMP build (oracle/_ref, when present;
                 else the oracle restatement) timed on this host's cores on a
                 bounded sample of the same workload;
@@ -41,7 +42,11 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_SCIDDICAT = 24.0   # z read, h read, h written (FP64)
+# The bytes a full-grid SciddicaT launch must move per cell, whether it
+# advances one step or two: h read once and written once (FP64).  z (8 B
+# more) is needed only where a wet cell is within reach of the launch's
+# steps; the fused sweeps read it only there.
+MIN_BYTES_PER_CELL_SCIDDICAT = 16.0
 SWEEP_PROFILE = os.path.join(ROOT, "profiles", "sciddicat_sweep_4096x4096.md")
 SWEEP2_PROFILE = os.path.join(ROOT, "profiles", "sciddicat_sweep2x_4096x4096.md")
 
@@ -96,13 +101,10 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024, batched=True):
     kavg = ktime_ms / max(launches, 1)
     return {
         "wall_s": wall, "gcups": cells * steps / wall / 1e9, "kernel_ms_avg": kavg, "kernel_launches": launches,
-        # the bytes one launch must move at least (z and h read once, h
-        # written once: 24 B per cell, whether the launch advances one step
-        # or two) over the average launch time
-        "achieved_gbs": BYTES_PER_CELL_SCIDDICAT * cells / (kavg * 1e-3) / 1e9,
-        # 24 B per cell update over the same time: a bound only for a
-        # one-step launch (a two-step launch reads and writes once for two)
-        "per_update_gbs": BYTES_PER_CELL_SCIDDICAT * cells * steps / (ktime_ms * 1e-3) / 1e9,
+        # the bytes one launch must move at least (h read once and written
+        # once: 16 B per cell, whether the launch advances one step or two)
+        # over the average launch time
+        "achieved_gbs": MIN_BYTES_PER_CELL_SCIDDICAT * cells / (kavg * 1e-3) / 1e9,
         "steps_per_launch": steps / max(launches, 1),
         "launches_per_step": st["launches"] / max(st["steps"], 1), "mass_rel_err": mass_err,
         "uploads": st["uploads"],
@@ -439,14 +441,13 @@ def main():
                                        "WRITE_SIZE, per two-step launch)",
                      "kernel": "step_fused_2x", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
                      "steps_per_launch": round(r["steps_per_launch"], 4),
-                     "algorithmic_bytes_per_launch": int(BYTES_PER_CELL_SCIDDICAT * args.size * args.size),
-                     "per_update_gbs": round(r["per_update_gbs"], 1),
-                     "note": "achieved = the bytes one launch must move (z and h read once, h written once: "
-                             "24 B per cell; the launch advances two steps and keeps the intermediate level in "
-                             "registers) / the launch's average duration from HIP events on the model's stream; "
-                             "per_update_gbs = 24 B per cell update over the same time, not a bound for a "
-                             "launch that advances two steps; traffic = measured HBM bytes per launch "
-                             "(rocprofv3 counters), traffic_gbs / traffic_frac over the same time"},
+                     "algorithmic_bytes_per_launch": int(MIN_BYTES_PER_CELL_SCIDDICAT * args.size * args.size),
+                     "note": "achieved = the bytes one launch must move (h read once and written once: 16 B "
+                             "per cell; the launch advances two steps and keeps the intermediate level in "
+                             "registers; z, 8 B more, is read only for strip chunks with a wet cell within two "
+                             "cells) / the launch's average duration from HIP events on the model's stream; "
+                             "traffic = measured HBM bytes per launch (rocprofv3 counters), traffic_gbs / "
+                             "traffic_frac over the same time"},
         "per_step_driver": {
             "driver": "calRunCAStep2D per step (one sweep per step)", "gcups": round(r1["gcups"], 3),
             "ms_per_step": round(r1["wall_s"] / args.steps * 1e3, 4),
@@ -455,7 +456,7 @@ def main():
                          "traffic": profiled_traffic() if args.size == 4096 else None,
                          "traffic_source": "profiles/sciddicat_sweep_4096x4096.md",
                          "kernel": "step_fused_al", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2),
-                         "algorithmic_bytes_per_launch": int(BYTES_PER_CELL_SCIDDICAT * args.size * args.size)}},
+                         "algorithmic_bytes_per_launch": int(MIN_BYTES_PER_CELL_SCIDDICAT * args.size * args.size)}},
     }
     # the measured HBM bytes per launch over the same kernel time: the rate the
     # memory actually sustains, beside the algorithmic rate of `achieved`
