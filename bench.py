@@ -153,26 +153,48 @@ def cpu_baseline_sciddicat(n, steps, seed=2024):
 # ----------------------------------------------------------- other configs
 
 def bench_active(n=1024, steps=4000, seed=7, cpu_steps=400):
-    """SciddicaT with CAL_OPT_ACTIVE_CELLS (config 2), stepped through the C API."""
+    """SciddicaT with CAL_OPT_ACTIVE_CELLS (config 2), stepped through the C
+    API two ways: one calRunCAStep2D per step (the reference example's loop,
+    CAL_RUN_LOOP with its stop condition: one launch per step) and
+    calhipRunSteps2D with the stop condition removed (the steps go to the
+    device program together: 4 steps per launch)."""
     import native
     from opencal_amd import synth
     from opencal_amd.models import SciddicaT
     z, h, _ = synth.sciddicat_inputs(n, n, seed)
-    m = SciddicaT(z, h, 0.5, 0.001, steps, active=True)
-    m.init()
-    m.steps(1)
-    m.sync()
-    t0 = time.perf_counter()
-    m.steps(steps - 1)
-    m.sync()
-    wall = time.perf_counter() - t0
-    _, gh, count = m.read()
-    m.close()
     _, oh, counts = native.oracle_sciddicat(z, h, 0.5, 0.001, steps, True, counts=True)
     active_updates = float(np.sum(counts[1:steps].astype(np.float64)))
-    out = {"workload": f"sciddicaT_active_{n}x{n}_x{steps}", "ms_per_step": wall / (steps - 1) * 1e3,
-           "active_gcups": active_updates / wall / 1e9, "full_grid_equiv_gcups": n * n * (steps - 1) / wall / 1e9,
-           "final_active_cells": count, "bit_exact_vs_oracle": bool(np.array_equal(gh, oh) and count == counts[-1])}
+    res = {}
+    for mode in ("steps", "run_steps"):
+        m = SciddicaT(z, h, 0.5, 0.001, steps, active=True)
+        m.init()
+        if mode == "run_steps":
+            m.clear_stop()
+        m.steps(1)
+        m.sync()
+        m.timing(True)
+        t0 = time.perf_counter()
+        if mode == "steps":
+            m.steps(steps - 1)
+        else:
+            m.run_steps(steps - 1)
+        m.sync()
+        wall = time.perf_counter() - t0
+        ktime_ms, launches = m.kernel_time()
+        m.timing(False)
+        _, gh, count = m.read()
+        m.close()
+        res[mode] = {"ms_per_step": wall / (steps - 1) * 1e3, "active_gcups": active_updates / wall / 1e9,
+                     "launches_per_step": launches / (steps - 1),
+                     "kernel_us_per_launch": ktime_ms / max(launches, 1) * 1e3,
+                     "bit_exact_vs_oracle": bool(np.array_equal(gh, oh) and count == counts[-1])}
+    r1, rk = res["steps"], res["run_steps"]
+    out = {"workload": f"sciddicaT_active_{n}x{n}_x{steps}", "ms_per_step": r1["ms_per_step"],
+           "active_gcups": r1["active_gcups"], "full_grid_equiv_gcups": n * n * (steps - 1) / (r1["ms_per_step"] * 1e-3 * (steps - 1)) / 1e9,
+           "driver": "calRunCAStep2D per step (CAL_RUN_LOOP with the stop condition, as the reference example)",
+           "per_step": r1, "run_steps": rk,
+           "final_active_cells": int(counts[-1]),
+           "bit_exact_vs_oracle": r1["bit_exact_vs_oracle"] and rk["bit_exact_vs_oracle"]}
     ref = native.ref("sciddicat_active")
     if ref is not None:
         t0 = time.perf_counter()

@@ -96,6 +96,7 @@ def _declare_examples(E):
         getattr(E, name).argtypes = [vp]
     E.sciddicat_steps.argtypes = [vp, C.c_int]
     E.sciddicat_run_steps.argtypes = [vp, C.c_int]
+    E.sciddicat_clear_stop.argtypes = [vp]
     E.sciddicat_read.argtypes = [vp, dp, dp, ip]
     E.sciddicat_set_explicit.argtypes = [vp, C.c_int]
     E.sciddicat_read_flow.argtypes = [vp, C.c_int, dp]

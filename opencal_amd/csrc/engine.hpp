@@ -92,6 +92,12 @@ struct Program {
 	// Called before the device active set is read back: restores the
 	// row-major order of the list when the program keeps it unordered.
 	virtual int sync_active(ModelX*) { return 0; }
+	// Called before a substate's current device layer is read as it is (a
+	// host read of one layer, a reduction): finishes deferred writes to it.
+	virtual int before_read(ModelX*) { return 0; }
+	// Launches every step the program has accepted but not yet launched
+	// (before a stream synchronisation, timing or statistics query).
+	virtual int flush(ModelX*) { return 0; }
 	// calApplyElementaryProcess2D/3D: one process (a CALCallbackFunc2D or
 	// CALCallbackFunc3D) over the grid (or the active list) without the
 	// update; 1 when the program offers no single-process sweep, <0 on error.

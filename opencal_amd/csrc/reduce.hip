@@ -235,6 +235,7 @@ T reduce_substate(ModelX* m, void* q, int op, const char* fn)
 	if (m->phase == Phase::Device && s->tstate != Transient::None && ensure_host(m)) return (T)0;
 	const size_t n = m->cells();
 	const bool on_device = m->phase == Phase::Device && !s->host_only;
+	if (on_device && m->prog && m->prog->before_read(m)) return (T)0;
 	if (n == 1 || op <= (int)REDUCTION_NONE || op > (int)REDUCTION_BINARY_XOR) {
 		// no fold: the value of cell 0 (cal2DReduction.c:30)
 		if (!on_device) return ((T*)s->h_cur())[0];

@@ -209,6 +209,7 @@ int host_read(ModelX* m, void* q)
 	if (m->phase == Phase::Host || s->host_only) return 0;
 	if (s->tstate != Transient::None) return ensure_host(m);   // logical content is not in the buffer
 	if (s->host_epoch == m->dev_epoch) return 0;
+	if (m->prog && m->prog->before_read(m)) return -1;
 	if (CALHIP_CHECK(hipMemcpyAsync(s->h_cur(), s->d_cur, m->cells() * (size_t)s->elem, hipMemcpyDeviceToHost,
 	                                m->stream)) ||
 	    CALHIP_CHECK(hipStreamSynchronize(m->stream)))
@@ -347,6 +348,7 @@ extern "C" int calhipSyncHost2D(struct CALModel2D* ca2D) { return ensure_host(X(
 extern "C" int calhipSynchronize2D(struct CALModel2D* ca2D)
 {
 	ModelX* m = X(ca2D);
+	if (m->prog && m->prog->flush(m)) return -1;
 	if (!m->stream) return 0;
 	return CALHIP_CHECK(hipStreamSynchronize(m->stream));
 }
@@ -355,13 +357,16 @@ extern "C" int calhipIsOnDevice2D(struct CALModel2D* ca2D) { return X(ca2D)->pha
 
 extern "C" int calhipGetStats2D(struct CALModel2D* ca2D, struct calhipStats* out)
 {
-	*out = X(ca2D)->stats;
+	ModelX* m = X(ca2D);
+	if (m->prog && m->prog->flush(m)) return -1;
+	*out = m->stats;
 	return 0;
 }
 
 extern "C" int calhipEnableKernelTiming2D(struct CALModel2D* ca2D, int enable)
 {
 	ModelX* m = X(ca2D);
+	if (m->prog && m->prog->flush(m)) return -1;
 	if (!enable) timer_collect(m);
 	m->timer.enabled = enable != 0;
 	if (enable) {
@@ -374,6 +379,7 @@ extern "C" int calhipEnableKernelTiming2D(struct CALModel2D* ca2D, int enable)
 extern "C" int calhipGetKernelTime2D(struct CALModel2D* ca2D, double* total_ms, long long* launches)
 {
 	ModelX* m = X(ca2D);
+	if (m->prog && m->prog->flush(m)) return -1;
 	timer_collect(m);
 	*total_ms = m->timer.total_ms;
 	*launches = m->timer.launches;

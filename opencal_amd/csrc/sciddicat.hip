@@ -49,6 +49,7 @@
 #include "engine.hpp"
 #include "device_common.hpp"
 #include "sciddicat_device.hpp"
+#include "sciddicat_active.hpp"
 
 namespace calhip {
 
@@ -445,6 +446,9 @@ struct SciddicaTProgram : Program {
 		}
 		set_flow_state(zero ? (active ? Transient::ZeroAtActive : Transient::Zero) : Transient::None);
 		h_prev_valid = false;
+		act_on = false;       // the upload replaced every buffer the mode kept
+		act_blocked = false;
+		act_pending = 0;
 		ctr_valid = false;    // the upload brought the list and its count
 		list_stale = false;
 		return 0;
@@ -465,14 +469,261 @@ struct SciddicaTProgram : Program {
 	bool ctr_valid = false;      // tiles.set_list[parity] lists the set's tiles
 	bool list_stale = false;     // m->d_cells / d_count lag behind the flags
 
+	// K steps per launch over tiles with halos (sciddicat_active.hpp): used
+	// for steps followed by the steering twin once the entry check passed.
+	struct ActBufs {
+		uint8_t* flags_b = nullptr;   // second flag copy (trades places with m->d_flags)
+		uint8_t* gh[2] = {nullptr, nullptr};
+		actf::Lists L = {};
+		char* block = nullptr;        // gh, tags, lists, counters
+		int ty = 0, tx = 0, n_tiles = 0, km = 0;
+	} act;
+	bool act_on = false;        // the mode's copies hold the state
+	bool act_blocked = false;   // the entry check failed; retried after the next upload
+	int act_gcur = 0;           // current ghost copy
+	int act_par = 0, act_rot = 0;
+	int act_last_mask = 2;      // L.mask index of the last launch's mask
+	unsigned act_epoch = 0;
+	int act_list_reach = 0;     // the current tile list covers launches of up to this many steps
+	int act_pending = 0;        // steps accepted by step_and_steer, not launched yet
+	int act_host_chk[2] = {0, 0};
+	unsigned act_host_init[6] = {0, 0, 0, 0, 0, 0};
+
 	~SciddicaTProgram() override
 	{
 		if (d_trash) hipFree(d_trash);
 		if (tiles.ctr) hipFree(tiles.ctr);
+		if (act.flags_b) hipFree(act.flags_b);
+		if (act.block) hipFree(act.block);
+	}
+
+	// The tiles' marking reach: at most KMAX, and no more than the rows
+	// (columns) of a last partial tile row (column) that sits between two
+	// others, so that a set cell's reach never skips a tile.
+	static int act_reach(int rows, int cols)
+	{
+		const int T = actf::TILE;
+		int km = actf::KMAX;
+		if ((rows + T - 1) / T >= 3 && rows % T) km = std::min(km, rows % T);
+		if ((cols + T - 1) / T >= 3 && cols % T) km = std::min(km, cols % T);
+		return km;
+	}
+
+	bool act_eligible(ModelX* m) const
+	{
+		if (!active || act_blocked || m->flags_dirty || getenv("CALHIP_SCIDDICAT_NO_FUSED_ACTIVE")) return false;
+		if (z->nxt_dirty || h->nxt_dirty) return false;
+		for (int q = 0; q < 4; ++q)
+			if ((f[q]->tstate != Transient::ZeroAtActive && f[q]->tstate != Transient::Zero) || f[q]->nxt_dirty)
+				return false;
+		return true;
+	}
+
+	actf::Launch act_launch_args(ModelX* m) const
+	{
+		actf::Launch p = {};
+		p.z = (const double*)z->d_cur;
+		p.h_in = (const double*)h->d_cur;
+		p.h_out = (double*)h->d_nxt;
+		p.fl_in = m->d_flags;
+		p.fl_out = act.flags_b;
+		p.gh_in = act.gh[act_gcur];
+		p.gh_out = act.gh[1 - act_gcur];
+		for (int q = 0; q < 4; ++q) {
+			p.F_in[q] = (const double*)f[q]->d_cur;
+			p.F_out[q] = (double*)f[q]->d_nxt;
+		}
+		p.L = act.L;
+		p.par = act_par;
+		p.rot = act_rot;
+		p.rows = m->pub.rows;
+		p.cols = m->pub.columns;
+		p.ty = act.ty;
+		p.tx = act.tx;
+		p.n_tiles = act.n_tiles;
+		p.km = act.km;
+		p.r = *p_r;
+		p.eps = *p_eps;
+		return p;
+	}
+
+	unsigned act_next_epoch(ModelX* m)
+	{
+		if (act_epoch >= 0xfffffff0u) {
+			// the tags would wrap: every tile back to "never listed"
+			if (CALHIP_CHECK(hipMemsetAsync(act.L.tag, 0, (size_t)act.n_tiles * sizeof(unsigned), m->stream)))
+				return 0;
+			act_epoch = 0;
+		}
+		return ++act_epoch;
+	}
+
+	// 1: the mode is on; 0: not eligible (or the entry check failed); -1: error.
+	int act_ready(ModelX* m)
+	{
+		if (act_on) return 1;
+		if (!act_eligible(m)) return 0;
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		const size_t n = m->cells();
+		if (!act.block) {
+			const int T = actf::TILE;
+			act.ty = (rows + T - 1) / T;
+			act.tx = (cols + T - 1) / T;
+			act.n_tiles = act.ty * act.tx;
+			act.km = act_reach(rows, cols);
+			const size_t nt = (size_t)act.n_tiles;
+			const size_t gh_bytes = (n + 255) & ~(size_t)255;
+			const size_t bytes = 2 * gh_bytes + nt * sizeof(unsigned) + 2 * nt * sizeof(int) + 8 * sizeof(int);
+			if (CALHIP_CHECK(hipMalloc(&act.flags_b, (n + 15) & ~(size_t)15))) return -1;
+			if (CALHIP_CHECK(hipMalloc(&act.block, bytes))) return -1;
+			if (CALHIP_CHECK(hipMemsetAsync(act.block, 0, bytes, m->stream))) return -1;
+			act.gh[0] = reinterpret_cast<uint8_t*>(act.block);
+			act.gh[1] = act.gh[0] + gh_bytes;
+			act.L.tag = reinterpret_cast<unsigned*>(act.gh[1] + gh_bytes);
+			act.L.list[0] = reinterpret_cast<int*>(act.L.tag + nt);
+			act.L.list[1] = act.L.list[0] + nt;
+			act.L.ctr = act.L.list[1] + nt;
+			act.L.mask = reinterpret_cast<unsigned*>(act.L.ctr + 3);
+			act.L.chk = act.L.ctr + 6;
+			act_epoch = 0;
+		}
+		// entry check: no cell outside the set with h > eps, a non-empty set;
+		// ghosts marked in gh[0]
+		bool f_zero = true;
+		const double* F[4];
+		for (int q = 0; q < 4; ++q) {
+			f_zero = f_zero && f[q]->tstate == Transient::Zero;
+			F[q] = (const double*)f[q]->d_cur;
+		}
+		if (CALHIP_CHECK(hipMemsetAsync(act.L.chk, 0, 2 * sizeof(int), m->stream))) return -1;
+		if (actf::launch_check(m->stream, (const double*)h->d_cur, m->d_flags, F, f_zero, act.gh[0], act.L.chk, n,
+		                       *p_eps))
+			return -1;
+		if (CALHIP_CHECK(hipMemcpyAsync(act_host_chk, act.L.chk, sizeof act_host_chk, hipMemcpyDeviceToHost,
+		                                m->stream)) ||
+		    CALHIP_CHECK(hipStreamSynchronize(m->stream)))
+			return -1;
+		m->stats.launches++;
+		if (act_host_chk[0] != 0 || act_host_chk[1] == 0) {
+			act_blocked = true;
+			return 0;
+		}
+		// counters 0, the "previous launch" of the first (index 2) left a
+		// non-empty set
+		for (int k = 0; k < 6; ++k) act_host_init[k] = 0;
+		act_host_init[3 + 2] = actf::MASK_SET_AFTER_LAST | actf::MASK_SET_BEFORE_LAST;
+		if (CALHIP_CHECK(hipMemcpyAsync(act.L.ctr, act_host_init, sizeof act_host_init, hipMemcpyHostToDevice,
+		                                m->stream)))
+			return -1;
+		act_gcur = 0;
+		act_par = 0;
+		act_rot = 0;
+		act_last_mask = 2;
+		actf::Launch p = act_launch_args(m);
+		if (!(p.e1 = act_next_epoch(m))) return -1;
+		if (actf::launch_setup(m->stream, p)) return -1;
+		m->stats.launches++;
+		act_list_reach = act.km;
+		act_on = true;
+		ctr_valid = false;
+		return 1;
+	}
+
+	static int act_pick(int lim) { return lim >= 4 ? 4 : (lim >= 2 ? 2 : 1); }
+
+	// n steps, each followed by the steering twin, in launches of K steps.
+	// A launch lists the tiles the next launch needs (reach: that launch's
+	// steps); the last one lists for as many steps as the grid allows, since
+	// the next call usually runs a group of them.
+	int act_steps(ModelX* m, int n)
+	{
+		const int grid = std::min(act.n_tiles, 2048);
+		while (n > 0) {
+			const int K = act_pick(std::min(n, act_list_reach));
+			const int Kn = act_pick(n > K ? std::min(n - K, act.km) : act.km);
+			actf::Launch p = act_launch_args(m);
+			p.km = Kn;
+			if (!(p.e1 = act_next_epoch(m))) return -1;
+			hipEvent_t ev;
+			timer_begin(m, &ev);
+			if (actf::launch_fused(m->stream, K, p, grid)) return -1;
+			timer_end(m, ev);
+			m->stats.launches++;
+			// the written copies become the current ones
+			flip(h);
+			for (int q = 0; q < 4; ++q) flip(f[q]);
+			std::swap(m->d_flags, act.flags_b);
+			act_gcur = 1 - act_gcur;
+			act_last_mask = act_rot;
+			act_par = 1 - act_par;
+			act_rot = (act_rot + 1) % 3;
+			act_list_reach = Kn;
+			n -= K;
+		}
+		set_flow_state(Transient::ZeroAtActive);
+		m->flags_dirty = false;
+		list_stale = true;
+		h_prev_valid = false;
+		return 0;
+	}
+
+	// Back to the ordinary representation: the flows' buffer holds their
+	// logical value outside the set (ZeroAtActive), h its final bits.
+	int act_leave(ModelX* m)
+	{
+		if (!act_on) return 0;
+		double* F[4];
+		for (int q = 0; q < 4; ++q) F[q] = (double*)f[q]->d_cur;
+		if (actf::launch_leave(m->stream, (double*)h->d_cur, m->d_flags, act.gh[act_gcur], F,
+		                       act.L.mask + act_last_mask, m->cells()))
+			return -1;
+		m->stats.launches++;
+		act_on = false;
+		ctr_valid = false;
+		list_stale = true;
+		return 0;
+	}
+
+	// Steps of calRunCAStep2D in the mode are launched in groups: a call
+	// records its step and every fourth launches them together.  Anything
+	// that can observe or change the state (every other hook of the
+	// program, host reads, reductions, synchronisation, timing and
+	// statistics queries) launches the recorded steps first.
+	int act_flush(ModelX* m)
+	{
+		if (!act_pending) return 0;
+		const int n = act_pending;
+		act_pending = 0;
+		return act_steps(m, n);
+	}
+
+	int flush(ModelX* m) override { return act_flush(m); }
+
+	int before_read(ModelX* m) override
+	{
+		if (act_flush(m)) return -1;
+		if (!act_on) return 0;
+		m->stats.launches++;
+		return actf::launch_normalise(m->stream, (double*)h->d_cur, act.L.mask + act_last_mask, m->cells());
+	}
+
+	int step_and_steer(ModelX* m) override
+	{
+		if (active) {
+			const int r = act_ready(m);
+			if (r < 0) return -1;
+			if (r) {
+				list_stale = true;
+				if (++act_pending >= actf::KMAX) return act_flush(m);
+				return 0;
+			}
+		}
+		return Program::step_and_steer(m);
 	}
 
 	int sync_active(ModelX* m) override
 	{
+		if (act_flush(m) || act_leave(m)) return -1;
 		if (!list_stale) return 0;
 		if (k::rebuild_active(m)) return -1;
 		m->stats.rebuilds++;
@@ -661,6 +912,7 @@ struct SciddicaTProgram : Program {
 
 	int step(ModelX* m) override
 	{
+		if (act_flush(m) || act_leave(m)) return -1;
 		if (check_entry(m)) return -1;
 		return active ? step_active(m) : step_full(m);
 	}
@@ -669,6 +921,7 @@ struct SciddicaTProgram : Program {
 	// variant resets the next layer only): the flows become logically zero.
 	int steer(ModelX* m) override
 	{
+		if (act_flush(m) || act_leave(m)) return -1;
 		for (int q = 0; q < 4; ++q)
 			if (device_fill(m, f[q], 0.0, !active)) return -1;   // calInitSubstate(Next)2Dr(f[q], 0)
 		return 0;
@@ -681,6 +934,16 @@ struct SciddicaTProgram : Program {
 	int run_steps(ModelX* m, int n, bool with_steering) override
 	{
 		const int rows = m->pub.rows, cols = m->pub.columns;
+		if (active && with_steering) {
+			const int r = act_ready(m);
+			if (r < 0) return -1;
+			if (r) {
+				n += act_pending;
+				act_pending = 0;
+				return act_steps(m, n);
+			}
+		}
+		if (act_flush(m)) return -1;
 		if (!with_steering || active || !sweep2_supported(rows, cols) || getenv("CALHIP_SCIDDICAT_NO_2X"))
 			return Program::run_steps(m, n, with_steering);
 		while (n >= 2) {
@@ -725,6 +988,7 @@ struct SciddicaTProgram : Program {
 			              "of the bound flows, width or remove");
 			return -1;
 		}
+		if (act_flush(m) || act_leave(m)) return -1;
 		for (int q = 0; q < 4; ++q)
 			if (f[q]->tstate != Transient::None) {
 				if (materialise(m, f[q])) return -1;
@@ -790,7 +1054,7 @@ struct SciddicaTProgram : Program {
 
 	int transient_init(ModelX* m, SubX* s, double v, bool both) override
 	{
-		(void)m;
+		if (act_flush(m)) return -1;
 		if (v != 0.0 || std::signbit(v)) return 0;
 		switch (s->tstate) {
 		case Transient::Zero:
@@ -822,6 +1086,7 @@ struct SciddicaTProgram : Program {
 	// and make it an ordinary substate.
 	int materialise(ModelX* m, SubX* s) override
 	{
+		if (act_flush(m) || act_leave(m)) return -1;
 		size_t n = m->cells();
 		int grid = stride_grid(n, 256);
 		switch (s->tstate) {
@@ -895,6 +1160,7 @@ struct SciddicaTProgram : Program {
 
 	int before_modify(ModelX* m, SubX* s) override
 	{
+		if (act_flush(m) || act_leave(m)) return -1;
 		if (s == h || s == z) {
 			for (int q = 0; q < 4; ++q)
 				if (f[q]->tstate == Transient::Derived || f[q]->tstate == Transient::PendingZero) {

@@ -1,0 +1,99 @@
+// SciddicaT active cells in one launch per K steps (K = 1, 2 or 4): the
+// kernels of sciddicat_active.hip and their launchers.
+//
+// Reference: the active-cells SciddicaT
+// (OpenCAL/examples/cal_sciddicaT-activecells-glut/source/sciddicaT.c:20-153)
+// on OpenCAL-OMP's engine (cal2D.c:333-440 set, :686-713 init / update of
+// the set's cells, :717-776 process sweeps and updates).  One step of the
+// reference, with the steering's flow reset (calInitSubstateNext2Dr at the
+// set's cells) after it, restated per cell.  A is the set at the start of
+// the step, F(c) the stored flows of cell c (zero at the cells of A: the
+// previous steering reset them there):
+//   out(c)  = the minimisation's outflows if c is in A and h(c) > eps,
+//             else F(c) (zero in A, otherwise what is stored);
+//   recv(c) = a wet cell of A kept c in its minimisation;
+//   h'(c)   = h + sum_n (out_{4-n}(neighbour n) - out_{n-1}(c)) for c in A
+//             or recv, else h;
+//   A'      = { c in A or recv : h'(c) > eps };
+//   F'(c)   = 0 in A' (steering), out(c) for cells of A that left the set,
+//             else F(c).
+// A cell outside A has h <= eps (checked when the mode starts: the init
+// adds every cell with h > 0, and a cell leaves only with h' <= eps), so a
+// wet cell is always in A.  Flows stored at a cell outside A are almost
+// always zero; the cells where they are not ("ghosts": a wet cell that left
+// the set, with r >= 1 or at a rounding edge, or flows the host wrote) carry
+// a ghost byte and keep their flows in the flow substates' buffers.
+//
+// Tiles of TILE x TILE cells.  A launch visits a list of tiles; each block
+// loads its tile with a halo of 2K cells (the step's reach is two cells:
+// the minimisation of a neighbour reads the neighbour's neighbours) and
+// advances it K steps in LDS, one cell per thread.  Every state buffer has
+// two copies (h and the flows in their substate's two layers, the set flags
+// and ghost bytes in two byte arrays): a launch reads one and writes the
+// other for every cell of its tiles, so no block reads what another block
+// writes.  A tile the launch does not visit holds equal copies.  The next
+// launch visits the tiles within KMAX cells of a cell of the new set and
+// the tiles whose contents changed (their other copy is now stale).
+//
+// When the set becomes empty OpenCAL's list is NULL and every process and
+// init covers the whole grid: the steering then resets every flow, and
+// every later step adds 0.0 to each thickness (turning -0.0 into +0.0) and
+// changes nothing else.  A launch that finds the previous one ended with an
+// empty set applies that to the whole grid; each launch records in its mask
+// whether the set was empty after its last and second-to-last steps, so a
+// read of the state can finish a run that emptied within its last launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace calhip {
+namespace actf {
+
+constexpr int TILE = 16;   // tile rows and columns
+constexpr int KMAX = 4;    // most steps per launch
+
+// mask bits of a launch
+constexpr unsigned MASK_SET_AFTER_LAST = 1u;    // the set is non-empty after the last step
+constexpr unsigned MASK_SET_BEFORE_LAST = 2u;   // ... after the second-to-last step (the start for K = 1)
+constexpr unsigned MASK_WHOLE_GRID = 4u;        // the launch found the set empty and covered the grid
+
+// Buffers shared by every launch of the mode.
+struct Lists {
+	unsigned* tag;      // [n_tiles] epoch of the last list the tile joined
+	int* list[2];       // [n_tiles] tiles to visit, by launch parity
+	int* ctr;           // [3] list lengths, by launch index mod 3
+	unsigned* mask;     // [3] launch masks, by launch index mod 3
+	int* chk;           // [2] entry check: cells outside the set with h > eps, cells in the set
+};
+
+// One launch: what it reads (_in) and writes (_out).
+struct Launch {
+	const double* z;
+	const double* h_in;
+	double* h_out;
+	const uint8_t* fl_in;
+	uint8_t* fl_out;
+	const uint8_t* gh_in;
+	uint8_t* gh_out;
+	const double* F_in[4];
+	double* F_out[4];
+	Lists L;
+	int par;            // list parity of this launch (reads list[par], fills list[1 - par])
+	int rot;            // launch index mod 3
+	unsigned e1;        // tag of the tiles listed for the next launch
+	int rows, cols, ty, tx, n_tiles;
+	int km;             // marking reach (>= the steps of the next launch)
+	double r, eps;
+};
+
+// Launchers (0 on success).  grid: blocks of the launch (<= n_tiles).
+int launch_check(hipStream_t st, const double* h, const uint8_t* fl, const double* const F[4], bool f_zero,
+                 uint8_t* gh, int* chk, size_t n, double eps);
+int launch_setup(hipStream_t st, const Launch& p);
+int launch_fused(hipStream_t st, int K, const Launch& p, int grid);
+int launch_leave(hipStream_t st, double* h, const uint8_t* fl, const uint8_t* gh, double* const F[4],
+                 const unsigned* mask_last, size_t n);
+int launch_normalise(hipStream_t st, double* h, const unsigned* mask_last, size_t n);
+
+}  // namespace actf
+}  // namespace calhip

@@ -203,6 +203,15 @@ int sciddicat_set_explicit(void* handle, int mode)
 	return 0;
 }
 
+/* Drop the stop condition (the active variant's step limit), so that
+ * calhipRunSteps2D may hand its steps to the device program together. */
+int sciddicat_clear_stop(void* handle)
+{
+	sciddicat_t* s = (sciddicat_t*)handle;
+	calRunAddStopConditionFunc2D(s->run, NULL);
+	return 0;
+}
+
 /* calRun2D: init, all steps, finalize (state back on the host afterwards). */
 int sciddicat_run(void* handle)
 {

@@ -54,6 +54,11 @@ class SciddicaT:
         with its steering twin advances two steps per sweep."""
         check(self.E.sciddicat_run_steps(self.handle, int(n)), "sciddicat_run_steps")
 
+    def clear_stop(self):
+        """Remove the active variant's stop condition (a host callback), so
+        that run_steps hands its steps to the device program together."""
+        check(self.E.sciddicat_clear_stop(self.handle), "sciddicat_clear_stop")
+
     def set_explicit(self, mode):
         """Run the processes one at a time from a global transition of the
         model's own: mode 1 with calUpdate2D after each, mode 2 (full grid)
