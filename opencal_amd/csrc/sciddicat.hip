@@ -477,6 +477,8 @@ struct SciddicaTProgram : Program {
 		actf::Lists L = {};
 		char* block = nullptr;        // gh, tags, lists, counters
 		int ty = 0, tx = 0, n_tiles = 0, km = 0;
+		int grid = 0;                 // blocks per launch: two per CU (a launch visits fewer tiles
+		                              // than that; blocks past the list's end exit at once)
 	} act;
 	bool act_on = false;        // the mode's copies hold the state
 	bool act_blocked = false;   // the entry check failed; retried after the next upload
@@ -571,6 +573,10 @@ struct SciddicaTProgram : Program {
 			act.tx = (cols + T - 1) / T;
 			act.n_tiles = act.ty * act.tx;
 			act.km = act_reach(rows, cols);
+			int cus = 0;
+			if (CALHIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->device)))
+				return -1;
+			act.grid = std::max(1, std::min(act.n_tiles, 2 * cus));
 			const size_t nt = (size_t)act.n_tiles;
 			const size_t gh_bytes = (n + 255) & ~(size_t)255;
 			const size_t bytes = 2 * gh_bytes + nt * sizeof(unsigned) + 2 * nt * sizeof(int) + 8 * sizeof(int);
@@ -637,7 +643,7 @@ struct SciddicaTProgram : Program {
 	// the next call usually runs a group of them.
 	int act_steps(ModelX* m, int n)
 	{
-		const int grid = std::min(act.n_tiles, 2048);
+		const int grid = act.grid;
 		while (n > 0) {
 			const int K = act_pick(std::min(n, act_list_reach));
 			const int Kn = act_pick(n > K ? std::min(n - K, act.km) : act.km);
