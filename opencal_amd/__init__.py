@@ -144,6 +144,11 @@ def _declare_examples(E):
     E.ggift_read.argtypes = [vp, ip, ip, dp]
     E.ggift_table.argtypes = [vp, ip, C.c_int]
     E.ggift_destroy.argtypes = [vp]
+    E.gmark_create.restype = vp
+    E.gmark_create.argtypes = [C.c_int, C.c_int, ip]
+    E.gmark_steps.argtypes = [vp, C.c_int]
+    E.gmark_read.argtypes = [vp, ip, C.c_int, ip]
+    E.gmark_destroy.argtypes = [vp]
     E.generic_check_sources.argtypes = [C.c_char_p, C.c_size_t]
     sbp = C.POINTER(C.c_byte)
     E.gred_create.restype = vp

@@ -622,7 +622,8 @@ void release_device(ModelX* x)
 	x->d_red = x->d_stage = nullptr;
 	x->stage_bytes = 0;
 	x->d_flags = nullptr;
-	x->d_cells = x->d_count = x->d_block_counts = nullptr;
+	x->d_cells = x->d_count = nullptr;
+	x->d_block_counts = nullptr;
 	for (auto& p : x->timer.pending) {
 		hipEventDestroy(p.first);
 		hipEventDestroy(p.second);

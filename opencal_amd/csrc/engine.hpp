@@ -152,8 +152,9 @@ struct ModelX {
 	uint8_t* d_flags = nullptr;
 	int* d_cells = nullptr;
 	int* d_count = nullptr;
-	int* d_block_counts = nullptr;
+	void* d_block_counts = nullptr;   // rebuild: chunk status words and counter (util.hip)
 	int n_count_blocks = 0;
+	unsigned rebuild_epoch = 0;
 	bool flags_dirty = false;  // flags changed since the last rebuild
 
 	Program* prog = nullptr;

@@ -367,6 +367,75 @@ void ggift_destroy(void* handle)
 	free(s);
 }
 
+/* ------------------------------------------------- active-set rebuild */
+
+CALHIP_SOURCE(gmark_source,
+struct gmark_state {
+	struct CALSubstate2Di* mark;   /* cells with a non-zero mark join the set */
+};
+CALHIP_GLOBAL struct gmark_state gmarkS;
+
+void gmark_select(struct CALModel2D* ca, int i, int j)
+{
+	if (calGet2Di(ca, gmarkS.mark, i, j))
+		calAddActiveCell2D(ca, i, j);
+	else
+		calRemoveActiveCell2D(ca, i, j);
+}
+)
+
+typedef struct {
+	struct CALModel2D* model;
+} gmark_t;
+
+/* An active-cells model whose one process adds the marked cells to the set
+ * (and removes the others); each transition ends with the device rebuild of
+ * the row-major list. */
+void* gmark_create(int rows, int cols, const int* mark)
+{
+	gmark_t* s = (gmark_t*)calloc(1, sizeof(gmark_t));
+	if (!s) return NULL;
+	s->model = calCADef2D(rows, cols, CAL_VON_NEUMANN_NEIGHBORHOOD_2D, CAL_SPACE_TOROIDAL, CAL_OPT_ACTIVE_CELLS);
+	if (!s->model) return NULL;
+	gmarkS.mark = calAddSubstate2Di(s->model);
+	memcpy(gmarkS.mark->current, mark, sizeof(int) * (size_t)rows * cols);
+	memcpy(gmarkS.mark->next, mark, sizeof(int) * (size_t)rows * cols);
+	calAddElementaryProcess2D(s->model, gmark_select);
+	if (calhipAddProcessSource2D(s->model, gmark_source) ||
+	    calhipBindProcess2D(s->model, gmark_select, "gmark_select") ||
+	    calhipBindGlobal2D(s->model, "gmarkS", &gmarkS, sizeof gmarkS))
+		return NULL;
+	return s;
+}
+
+int gmark_steps(void* handle, int n)
+{
+	gmark_t* s = (gmark_t*)handle;
+	int k;
+	for (k = 0; k < n; k++) calGlobalTransitionFunction2D(s->model);
+	return calhipGetLastError();
+}
+
+/* the set as OpenCAL holds it: A.cells as linear indices, A.size_current */
+int gmark_read(void* handle, int* cells, int max_cells, int* count)
+{
+	gmark_t* s = (gmark_t*)handle;
+	struct CALModel2D* m = s->model;
+	int k;
+	if (calhipSyncHost2D(m)) return -1;
+	*count = m->A.size_current;
+	for (k = 0; k < m->A.size_current && k < max_cells; k++) cells[k] = m->A.cells[k].i * m->columns + m->A.cells[k].j;
+	return calhipGetLastError();
+}
+
+void gmark_destroy(void* handle)
+{
+	gmark_t* s = (gmark_t*)handle;
+	if (!s) return;
+	calFinalize2D(s->model);
+	free(s);
+}
+
 /* ------------------------------------------------------------ reductions */
 
 CALHIP_SOURCE(gred_source,
@@ -460,5 +529,7 @@ int generic_check_sources(char* log, size_t log_bytes)
 	const char* red[] = {"gred_keep"};
 	if (calhipCheckProcessSource2D(ggift_source, gift, 2, log, log_bytes)) return 3;
 	if (calhipCheckProcessSource2D(gred_source, red, 1, log, log_bytes)) return 4;
+	const char* mark[] = {"gmark_select"};
+	if (calhipCheckProcessSource2D(gmark_source, mark, 1, log, log_bytes)) return 5;
 	return 0;
 }

@@ -214,3 +214,32 @@ def test_generic_unsafe_operations(nb, torus):
         assert np.array_equal(q, eq + eq_apply)
     finally:
         E.ggift_destroy(hd)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows,cols,density,seed", [
+    (1, 1, 1.0, 1), (3, 5, 0.5, 2), (37, 53, 0.3, 3), (64, 64, 0.0, 4), (64, 64, 1.0, 5),
+    (257, 129, 0.001, 6), (1000, 1000, 0.3, 7), (2048, 3001, 0.7, 8), (4096, 4096, 0.02, 9),
+])
+def test_active_rebuild_row_major(rows, cols, density, seed):
+    """The device rebuild of the active list (one launch: ballot ranks per
+    wave, look-back prefix across chunks) gives OpenCAL's row-major A.cells
+    and A.size_current for seeded flag patterns from empty to full, over
+    grids of one chunk to 4096 chunks, on three rebuilds in a row."""
+    print(f"case rebuild {rows}x{cols} density={density} seed={seed}", flush=True)
+    n = rows * cols
+    mark = (synth.uniform(seed, n) < density).astype(np.int32).reshape(rows, cols)
+    want = np.flatnonzero(mark)
+    E = examples()
+    h = E.gmark_create(rows, cols, mark.ctypes.data_as(_ip))
+    assert h
+    try:
+        for rep in range(3):
+            assert E.gmark_steps(h, 1) == 0
+            got = np.zeros(n + 1, np.int32)
+            cnt = C.c_int(0)
+            assert E.gmark_read(h, got.ctypes.data_as(_ip), n, C.byref(cnt)) == 0
+            assert cnt.value == len(want), f"rebuild {rep}"
+            assert np.array_equal(got[:cnt.value], want), f"rebuild {rep}"
+    finally:
+        E.gmark_destroy(h)
