@@ -449,6 +449,8 @@ struct SciddicaTProgram : Program {
 		act_on = false;       // the upload replaced every buffer the mode kept
 		act_blocked = false;
 		act_pending = 0;
+		pl_valid = false;
+		pl_declined = false;
 		ctr_valid = false;    // the upload brought the list and its count
 		list_stale = false;
 		return 0;
@@ -488,6 +490,26 @@ struct SciddicaTProgram : Program {
 	unsigned act_epoch = 0;
 	int act_list_reach = 0;     // the current tile list covers launches of up to this many steps
 	int act_pending = 0;        // steps accepted by step_and_steer, not launched yet
+
+	// The full grid in launches of up to PL_KMAX steps over the same tiles
+	// (sciddicat_active.hpp), for steps followed by the steering twin while
+	// few tiles are near wet cells.
+	static constexpr int PL_KMAX = 8;
+	struct PlainBufs {
+		unsigned* tag[2] = {nullptr, nullptr};
+		int* list[2] = {nullptr, nullptr};
+		int* cnt = nullptr;        // [3] list lengths by launch index mod 3
+		char* block = nullptr;
+		int* h_cnt = nullptr;      // pinned copy of a list length (read without a stream sync)
+		hipEvent_t ev = nullptr;
+		int ty = 0, tx = 0, n_tiles = 0, km = 0, cus = 0;
+	} pl;
+	bool pl_valid = false;       // pl.list[pl_L % 2] lists the tiles the next launch needs
+	bool pl_declined = false;    // too many tiles near wet cells for the mode to pay
+	int pl_L = 0;                // launch index since the list was built
+	unsigned pl_e = 0;           // tag of the current list
+	int pl_reach = 0;            // the current list covers launches of up to this many steps
+	bool pl_cnt_pending = false; // h_cnt awaits ev
 	int act_host_chk[2] = {0, 0};
 	unsigned act_host_init[6] = {0, 0, 0, 0, 0, 0};
 
@@ -497,6 +519,9 @@ struct SciddicaTProgram : Program {
 		if (tiles.ctr) hipFree(tiles.ctr);
 		if (act.flags_b) hipFree(act.flags_b);
 		if (act.block) hipFree(act.block);
+		if (pl.block) hipFree(pl.block);
+		if (pl.h_cnt) (void)hipHostFree(pl.h_cnt);
+		if (pl.ev) (void)hipEventDestroy(pl.ev);
 	}
 
 	// The tiles' marking reach: at most KMAX, and no more than the rows
@@ -700,7 +725,11 @@ struct SciddicaTProgram : Program {
 		if (!act_pending) return 0;
 		const int n = act_pending;
 		act_pending = 0;
-		return act_steps(m, n);
+		if (active) return act_steps(m, n);
+		// full grid: the steps the tile mode does not take run one at a time
+		const int done = pl_run(m, n);
+		if (done < 0) return -1;
+		return done < n ? Program::run_steps(m, n - done, true) : 0;
 	}
 
 	int flush(ModelX* m) override { return act_flush(m); }
@@ -723,8 +752,162 @@ struct SciddicaTProgram : Program {
 				if (++act_pending >= actf::KMAX) return act_flush(m);
 				return 0;
 			}
+		} else if (!act_pending) {
+			const int r = pl_ready(m);
+			if (r < 0) return -1;
+			if (r) {
+				++act_pending;
+				return 0;
+			}
+		} else {
+			if (++act_pending >= PL_KMAX) return act_flush(m);
+			return 0;
 		}
 		return Program::step_and_steer(m);
+	}
+
+	// ---- full grid in launches of K steps ----
+
+	bool pl_eligible(ModelX* m) const
+	{
+		if (active || pl_declined || getenv("CALHIP_SCIDDICAT_NO_TILES")) return false;
+		if (z->nxt_dirty || h->nxt_dirty) return false;
+		for (int q = 0; q < 4; ++q)
+			if (f[q]->tstate != Transient::Zero || f[q]->nxt_dirty) return false;
+		(void)m;
+		return true;
+	}
+
+	// 1: the tile list is built and the mode pays; 0: not used; -1: error.
+	int pl_ready(ModelX* m)
+	{
+		if (pl_valid) return 1;
+		if (!pl_eligible(m)) return 0;
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		if (!pl.block) {
+			const int T = actf::TILE;
+			pl.ty = (rows + T - 1) / T;
+			pl.tx = (cols + T - 1) / T;
+			pl.n_tiles = pl.ty * pl.tx;
+			int km = PL_KMAX;
+			if (pl.ty >= 3 && rows % T) km = std::min(km, rows % T);
+			if (pl.tx >= 3 && cols % T) km = std::min(km, cols % T);
+			pl.km = km;
+			if (CALHIP_CHECK(hipDeviceGetAttribute(&pl.cus, hipDeviceAttributeMultiprocessorCount, m->device)))
+				return -1;
+			const size_t nt = (size_t)pl.n_tiles;
+			const size_t bytes = 2 * nt * sizeof(unsigned) + 2 * nt * sizeof(int) + 4 * sizeof(int);
+			if (CALHIP_CHECK(hipMalloc(&pl.block, bytes))) return -1;
+			if (CALHIP_CHECK(hipMemsetAsync(pl.block, 0, bytes, m->stream))) return -1;
+			pl.tag[0] = reinterpret_cast<unsigned*>(pl.block);
+			pl.tag[1] = pl.tag[0] + nt;
+			pl.list[0] = reinterpret_cast<int*>(pl.tag[1] + nt);
+			pl.list[1] = pl.list[0] + nt;
+			pl.cnt = pl.list[1] + nt;
+			if (CALHIP_CHECK(hipHostMalloc((void**)&pl.h_cnt, sizeof(int), hipHostMallocDefault))) return -1;
+			if (CALHIP_CHECK(hipEventCreateWithFlags(&pl.ev, hipEventDisableTiming))) return -1;
+			pl_e = 0;
+		}
+		if (pl_e >= 0xfffffff0u) {
+			// the tags would wrap: every tile back to "never listed"
+			if (CALHIP_CHECK(hipMemsetAsync(pl.tag[0], 0, 2 * (size_t)pl.n_tiles * sizeof(unsigned), m->stream)))
+				return -1;
+			pl_e = 0;
+		}
+		if (CALHIP_CHECK(hipMemsetAsync(pl.cnt, 0, 3 * sizeof(int), m->stream))) return -1;
+		pl_L = 0;
+		actf::PlainLaunch p = pl_args(m);
+		p.e = ++pl_e;
+		p.tag_out = pl.tag[0];
+		p.list_out = pl.list[0];
+		p.cnt_out = pl.cnt;
+		p.km = pl.km;
+		if (actf::launch_plain_setup(m->stream, p)) return -1;
+		m->stats.launches++;
+		if (CALHIP_CHECK(hipMemcpyAsync(pl.h_cnt, pl.cnt, sizeof(int), hipMemcpyDeviceToHost, m->stream)) ||
+		    CALHIP_CHECK(hipStreamSynchronize(m->stream)))
+			return -1;
+		pl_cnt_pending = false;
+		// CALHIP_SCIDDICAT_TILES_ALWAYS keeps the mode on whatever the wet area
+		const bool always = getenv("CALHIP_SCIDDICAT_TILES_ALWAYS") != nullptr;
+		if (!always && *pl.h_cnt > pl.n_tiles / 8) {
+			pl_declined = true;
+			return 0;
+		}
+		pl_valid = true;
+		pl_reach = pl.km;
+		return 1;
+	}
+
+	actf::PlainLaunch pl_args(ModelX* m) const
+	{
+		actf::PlainLaunch p = {};
+		p.z = (const double*)z->d_cur;
+		p.h_in = (const double*)h->d_cur;
+		p.h_out = (double*)h->d_nxt;
+		p.tag_in = pl.tag[pl_L % 2];
+		p.tag_out = pl.tag[(pl_L + 1) % 2];
+		p.list_in = pl.list[pl_L % 2];
+		p.list_out = pl.list[(pl_L + 1) % 2];
+		p.cnt_in = pl.cnt + pl_L % 3;
+		p.cnt_out = pl.cnt + (pl_L + 1) % 3;
+		p.cnt_clear = pl.cnt + (pl_L + 2) % 3;
+		p.e = pl_e;
+		p.rows = m->pub.rows;
+		p.cols = m->pub.columns;
+		p.ty = pl.ty;
+		p.tx = pl.tx;
+		p.n_tiles = pl.n_tiles;
+		p.r = *p_r;
+		p.eps = *p_eps;
+		return p;
+	}
+
+	static int pl_pick(int lim) { return lim >= 8 ? 8 : (lim >= 4 ? 4 : (lim >= 2 ? 2 : 1)); }
+
+	// Up to n steps of the full grid (each followed by the steering twin);
+	// returns the steps taken: fewer than n when the mode stops paying (too
+	// many tiles near wet cells), -1 on error.
+	int pl_run(ModelX* m, int n)
+	{
+		const int r0 = pl_ready(m);
+		if (r0 <= 0) return r0 < 0 ? -1 : 0;
+		int done = 0;
+		while (done < n) {
+			if (pl_cnt_pending && hipEventQuery(pl.ev) == hipSuccess) {
+				pl_cnt_pending = false;
+				if (*pl.h_cnt > pl.n_tiles / 4 && !getenv("CALHIP_SCIDDICAT_TILES_ALWAYS")) {
+					pl_valid = false;
+					pl_declined = true;
+					break;
+				}
+			}
+			const int K = pl_pick(std::min(n - done, pl_reach));
+			const int Kn = pl_pick(n - done > K ? std::min(n - done - K, pl.km) : pl.km);
+			actf::PlainLaunch p = pl_args(m);
+			p.km = Kn;
+			p.grid_comp = std::max(1, std::min(pl.n_tiles, pl.cus * (K == 8 ? 1 : 2)));
+			hipEvent_t ev;
+			timer_begin(m, &ev);
+			if (actf::launch_plain(m->stream, K, p)) return -1;
+			timer_end(m, ev);
+			m->stats.launches++;
+			flip(h);
+			pl_L++;
+			pl_e++;
+			pl_reach = Kn;
+			done += K;
+			if ((pl_L & 7) == 0 && !pl_cnt_pending) {
+				if (CALHIP_CHECK(hipMemcpyAsync(pl.h_cnt, pl.cnt + pl_L % 3, sizeof(int), hipMemcpyDeviceToHost,
+				                                m->stream)) ||
+				    CALHIP_CHECK(hipEventRecord(pl.ev, m->stream)))
+					return -1;
+				pl_cnt_pending = true;
+			}
+		}
+		set_flow_state(Transient::Zero);
+		h_prev_valid = false;
+		return done;
 	}
 
 	int sync_active(ModelX* m) override
@@ -919,6 +1102,7 @@ struct SciddicaTProgram : Program {
 	int step(ModelX* m) override
 	{
 		if (act_flush(m) || act_leave(m)) return -1;
+		pl_valid = false;
 		if (check_entry(m)) return -1;
 		return active ? step_active(m) : step_full(m);
 	}
@@ -948,6 +1132,14 @@ struct SciddicaTProgram : Program {
 				act_pending = 0;
 				return act_steps(m, n);
 			}
+		}
+		if (!active && with_steering) {
+			n += act_pending;
+			act_pending = 0;
+			const int done = pl_run(m, n);
+			if (done < 0) return -1;
+			n -= done;
+			if (n == 0) return 0;
 		}
 		if (act_flush(m)) return -1;
 		if (!with_steering || active || !sweep2_supported(rows, cols) || getenv("CALHIP_SCIDDICAT_NO_2X"))
@@ -995,6 +1187,7 @@ struct SciddicaTProgram : Program {
 			return -1;
 		}
 		if (act_flush(m) || act_leave(m)) return -1;
+		pl_valid = false;
 		for (int q = 0; q < 4; ++q)
 			if (f[q]->tstate != Transient::None) {
 				if (materialise(m, f[q])) return -1;
@@ -1167,6 +1360,10 @@ struct SciddicaTProgram : Program {
 	int before_modify(ModelX* m, SubX* s) override
 	{
 		if (act_flush(m) || act_leave(m)) return -1;
+		if (s == h || s == z) {
+			pl_valid = false;
+			pl_declined = false;
+		}
 		if (s == h || s == z) {
 			for (int q = 0; q < 4; ++q)
 				if (f[q]->tstate == Transient::Derived || f[q]->tstate == Transient::PendingZero) {

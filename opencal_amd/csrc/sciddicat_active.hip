@@ -311,6 +311,162 @@ __global__ __launch_bounds__(256) void act_normalise(double* __restrict__ h, con
 	}
 }
 
+// ---- full grid ----
+
+// Cells per thread of the K-step full-grid kernel: the (16 + 4K)^2 region
+// over at most 1024 threads.
+template <int K>
+struct PlainShape {
+	static constexpr int RW = TILE + 4 * K, NC = RW * RW;
+	static constexpr int CPT = (NC + 1023) / 1024;
+	static constexpr int NT = (NC + CPT - 1) / CPT;
+};
+
+// Lists every tile within p.km of a wet cell (h > eps or NaN) of the grid in
+// list_out with tag e; cnt_out is zero on entry.
+__global__ __launch_bounds__(TILE * TILE) void plain_setup(PlainLaunch p)
+{
+	__shared__ unsigned sh_mark;
+	const int tid = threadIdx.x, la = tid / TILE, lb = tid % TILE;
+	for (int t = blockIdx.x; t < p.n_tiles; t += gridDim.x) {
+		if (tid == 0) sh_mark = 0u;
+		__syncthreads();
+		const int ti = t % p.ty, tj = t / p.ty;
+		const int tre = min(TILE, p.rows - ti * TILE), tce = min(TILE, p.cols - tj * TILE);
+		if (la < tre && lb < tce) {
+			const size_t c = (size_t)(ti * TILE + la) * p.cols + tj * TILE + lb;
+			if (!(p.h_in[c] <= p.eps)) atomicOr(&sh_mark, reach_marks(la, lb, tre, tce, p.km));
+		}
+		__syncthreads();
+		const unsigned marks = sh_mark;
+		__syncthreads();
+		list_marked(marks, tid, ti, tj, p.ty, p.tx, p.tag_out, p.e, p.list_out, p.cnt_out);
+	}
+}
+
+// Blocks [0, grid_comp): K steps of the listed tiles.  The others: the
+// copy h + 0.0 of every unlisted tile, each block a band of 16 rows (one
+// tile row) and 2 * NT columns, two cells per thread and row.
+template <int K>
+__global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
+{
+	using S = PlainShape<K>;
+	constexpr int H = 2 * K, RW = S::RW, NC = S::NC, CPT = S::CPT, NT = S::NT;
+	__shared__ double su[NC];
+	__shared__ double so[4][NC];
+	__shared__ unsigned sh_mark;
+	const int tid = threadIdx.x;
+	if ((int)blockIdx.x >= p.grid_comp) {
+		// copy role
+		const int band = (int)blockIdx.x - p.grid_comp;
+		const int per_row = (p.cols + 2 * NT - 1) / (2 * NT);
+		const int ti = band / per_row, bc = band - ti * per_row;
+		if (ti >= p.ty) return;
+		const int j = bc * 2 * NT + 2 * tid;
+		if (j >= p.cols) return;
+		const int tj0 = j / TILE, tj1 = (j + 1) / TILE;
+		const bool c0 = p.tag_in[ti + tj0 * p.ty] != p.e;
+		const bool c1 = j + 1 < p.cols && p.tag_in[ti + tj1 * p.ty] != p.e;
+		const int i1 = min(p.rows, (ti + 1) * TILE);
+		const bool pair = c0 && c1 && (p.cols % 2 == 0);   // j even: 16-byte aligned pair
+		for (int i = ti * TILE; i < i1; ++i) {
+			const size_t c = (size_t)i * p.cols + j;
+			if (pair) {
+				double2 v = *reinterpret_cast<const double2*>(p.h_in + c);
+				v.x += 0.0;   // h + (0.0 - 0.0) x 4: every flow around the cell is zero
+				v.y += 0.0;
+				*reinterpret_cast<double2*>(p.h_out + c) = v;
+			} else {
+				if (c0) p.h_out[c] = p.h_in[c] + 0.0;
+				if (c1) p.h_out[c + 1] = p.h_in[c + 1] + 0.0;
+			}
+		}
+		return;
+	}
+	if (blockIdx.x == 0 && tid == 0) *p.cnt_clear = 0;
+	const int n_in = *p.cnt_in;
+	const double r = p.r, eps = p.eps;
+	for (int k = blockIdx.x; k < n_in; k += p.grid_comp) {
+		const int t = p.list_in[k];
+		const int ti = t % p.ty, tj = t / p.ty;
+		if (tid == 0) sh_mark = 0u;
+		double z[CPT], h[CPT];
+#pragma unroll
+		for (int u = 0; u < CPT; ++u) {
+			const int q = tid + u * NT;
+			const int a = q / RW, b = q - (q / RW) * RW;
+			z[u] = 0.0;
+			h[u] = 0.0;
+			if (q < NC) {
+				const size_t c = (size_t)wrapm(ti * TILE - H + a, p.rows) * p.cols + wrapm(tj * TILE - H + b, p.cols);
+				h[u] = p.h_in[c];
+				z[u] = p.z[c];
+			}
+		}
+#pragma unroll 1
+		for (int st = 0; st < K; ++st) {
+			const int lo = 2 * st, hi = RW - 2 * st;
+#pragma unroll
+			for (int u = 0; u < CPT; ++u) {
+				const int q = tid + u * NT;
+				const int a = q / RW, b = q - (q / RW) * RW;
+				if (q < NC && a >= lo && a < hi && b >= lo && b < hi) su[q] = z[u] + h[u];
+			}
+			__syncthreads();
+#pragma unroll
+			for (int u = 0; u < CPT; ++u) {
+				const int q = tid + u * NT;
+				const int a = q / RW, b = q - (q / RW) * RW;
+				if (q < NC && a >= lo + 1 && a < hi - 1 && b >= lo + 1 && b < hi - 1) {
+					double o[4] = {0.0, 0.0, 0.0, 0.0};
+					if (!(h[u] <= eps)) {
+						const FlowOut fo = flows_wet_body(h[u], z[u], su[q - RW], su[q - 1], su[q + 1], su[q + RW], r, eps);
+						o[0] = fo.f0;
+						o[1] = fo.f1;
+						o[2] = fo.f2;
+						o[3] = fo.f3;
+					}
+#pragma unroll
+					for (int d = 0; d < 4; ++d) so[d][q] = o[d];
+				}
+			}
+			__syncthreads();
+#pragma unroll
+			for (int u = 0; u < CPT; ++u) {
+				const int q = tid + u * NT;
+				const int a = q / RW, b = q - (q / RW) * RW;
+				if (q < NC && a >= lo + 2 && a < hi - 2 && b >= lo + 2 && b < hi - 2) {
+					double v = h[u];
+					v += so[3][q - RW] - so[0][q];
+					v += so[2][q - 1] - so[1][q];
+					v += so[1][q + 1] - so[2][q];
+					v += so[0][q + RW] - so[3][q];
+					h[u] = v;
+				}
+			}
+			// the next step writes su and so only after its first barrier,
+			// when every read of this step is done
+		}
+		const int tre = min(TILE, p.rows - ti * TILE), tce = min(TILE, p.cols - tj * TILE);
+		unsigned mk = 0u;
+#pragma unroll
+		for (int u = 0; u < CPT; ++u) {
+			const int q = tid + u * NT;
+			const int la = q / RW - H, lb = q - (q / RW) * RW - H;
+			if (q < NC && la >= 0 && la < tre && lb >= 0 && lb < tce) {
+				const size_t c = (size_t)(ti * TILE + la) * p.cols + tj * TILE + lb;
+				p.h_out[c] = h[u];
+				if (!(h[u] <= eps)) mk |= reach_marks(la, lb, tre, tce, p.km);
+			}
+		}
+		if (mk) atomicOr(&sh_mark, mk);
+		__syncthreads();
+		const unsigned marks = sh_mark;
+		__syncthreads();   // read before the next tile clears it
+		list_marked(marks, tid, ti, tj, p.ty, p.tx, p.tag_out, p.e + 1u, p.list_out, p.cnt_out);
+	}
+}
+
 int grid_for(size_t n)
 {
 	const size_t b = (n + 255) / 256;
@@ -358,6 +514,47 @@ int launch_leave(hipStream_t st, double* h, const uint8_t* fl, const uint8_t* gh
 {
 	hipLaunchKernelGGL(act_leave, dim3(grid_for(n)), dim3(256), 0, st, h, fl, gh, F[0], F[1], F[2], F[3],
 	                   mask_last, n);
+	return CALHIP_CHECK(hipGetLastError());
+}
+
+int plain_threads(int K)
+{
+	return K == 8 ? PlainShape<8>::NT : (K == 4 ? PlainShape<4>::NT : (K == 2 ? PlainShape<2>::NT : PlainShape<1>::NT));
+}
+
+int plain_copy_blocks(int K, int rows, int cols)
+{
+	const int nt = plain_threads(K);
+	return ((rows + TILE - 1) / TILE) * ((cols + 2 * nt - 1) / (2 * nt));
+}
+
+int launch_plain_setup(hipStream_t st, const PlainLaunch& p)
+{
+	const int grid = p.n_tiles < 4096 ? p.n_tiles : 4096;
+	hipLaunchKernelGGL(plain_setup, dim3(grid), dim3(TILE * TILE), 0, st, p);
+	return CALHIP_CHECK(hipGetLastError());
+}
+
+int launch_plain(hipStream_t st, int K, const PlainLaunch& p)
+{
+	const int grid = p.grid_comp + plain_copy_blocks(K, p.rows, p.cols);
+	switch (K) {
+	case 1:
+		hipLaunchKernelGGL(plain_tiles<1>, dim3(grid), dim3(PlainShape<1>::NT), 0, st, p);
+		break;
+	case 2:
+		hipLaunchKernelGGL(plain_tiles<2>, dim3(grid), dim3(PlainShape<2>::NT), 0, st, p);
+		break;
+	case 4:
+		hipLaunchKernelGGL(plain_tiles<4>, dim3(grid), dim3(PlainShape<4>::NT), 0, st, p);
+		break;
+	case 8:
+		hipLaunchKernelGGL(plain_tiles<8>, dim3(grid), dim3(PlainShape<8>::NT), 0, st, p);
+		break;
+	default:
+		set_error(13, "SciddicaT full-grid launch: steps per launch must be 1, 2, 4 or 8");
+		return -1;
+	}
 	return CALHIP_CHECK(hipGetLastError());
 }
 

@@ -95,5 +95,40 @@ int launch_leave(hipStream_t st, double* h, const uint8_t* fl, const uint8_t* gh
                  const unsigned* mask_last, size_t n);
 int launch_normalise(hipStream_t st, double* h, const unsigned* mask_last, size_t n);
 
+// ---- full grid (CAL_NO_OPT) in launches of K steps ----
+//
+// The same tiles for the full-grid model with the steering's flow reset
+// between steps: every cell is processed every step, a cell with h > eps
+// (or NaN) sends its outflows, and a dry cell whose neighbours are all dry
+// only adds 0.0 to its thickness (turning -0.0 into +0.0).  A launch
+// advances the grid K steps: the listed tiles (every tile within reach of a
+// wet cell) in LDS as above, every other tile by a streaming copy h + 0.0,
+// both from one layer into the other, so each launch reads and writes every
+// cell once.  The listed tiles carry the current launch's tag; the launch
+// marks the next launch's tiles in the other tag array.
+struct PlainLaunch {
+	const double* z;
+	const double* h_in;
+	double* h_out;
+	const unsigned* tag_in;    // tiles of this launch: tag_in[t] == e
+	unsigned* tag_out;         // tiles of the next launch get tag e + 1
+	const int* list_in;
+	int* list_out;
+	const int* cnt_in;
+	int* cnt_out;              // zero on entry
+	int* cnt_clear;            // zeroed by the launch (the next launch's cnt_out)
+	unsigned e;
+	int rows, cols, ty, tx, n_tiles;
+	int km;                    // marking reach: the next launch's steps
+	int grid_comp;             // blocks that visit the list; the rest copy
+	double r, eps;
+};
+
+// threads per block of the K-step kernel and the column width of one copy block
+int plain_threads(int K);
+int plain_copy_blocks(int K, int rows, int cols);
+int launch_plain_setup(hipStream_t st, const PlainLaunch& p);
+int launch_plain(hipStream_t st, int K, const PlainLaunch& p);
+
 }  // namespace actf
 }  // namespace calhip
