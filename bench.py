@@ -6,21 +6,26 @@ example drives it: the steps of calRun2D's loop (calhipRunSteps2D: global
 transition, the model's steering — the reference flow reset, declared with
 calhipBindSciddicaTSteering2D so that its device twin runs — and the implicit
 update), with the model already resident in HBM.  Between two steps the
-flows are reset, so the engine advances the grid two steps per sweep.  The
-same workload with one calRunCAStep2D call (one sweep) per step is reported
+flows are reset, so a step is a function of z and h alone and the engine
+advances the grid up to eight steps per launch: the tiles within reach of a
+wet cell in LDS, every other cell by a streaming copy, each launch reading
+and writing every cell once.  The same workload with one calRunCAStep2D
+call per step (which the engine groups eight to a launch) is reported
 beside it (`per_step_driver`).  Unit: giga cell updates per second.
 
 The same JSON line carries
-  roofline      the fused sweep kernel's achieved algorithmic bandwidth
-                (16 B per cell and launch: h read, h written; z is read only
-                near wet cells) from HIP events on the model's stream, against
-                the 8 TB/s HBM peak, and its measured HBM traffic;
+  roofline      the full-grid kernel's achieved algorithmic bandwidth (16 B
+                per cell and launch: h read, h written; z is read only in the
+                tiles near wet cells) from HIP events on the model's stream,
+                against the 8 TB/s HBM peak, and its measured HBM traffic;
   cpu_baseline  the reference OpenCAL-OMP build (oracle/_ref, when present;
                 else the oracle restatement) timed on this host's cores on a
                 bounded sample of the same workload;
   workloads     the other measured configurations: SciddicaT active cells
-                1024^2 x 4000, Life 512^2 x 1000, the batched ensemble of 256
-                uneven instances x 2000 steps against the same instances run
+                1024^2 x 4000, the same 4096^2 full grid with debris on
+                every cell (the minimisation everywhere), Life 512^2 x 1000,
+                the batched ensemble of 256 uneven instances x 2000 steps
+                against the same instances run
                 one after another, and the adjoint gradient / forward time
                 ratio at 1024^2 x 2000 with checkpoints every 45 steps, and
                 SCIARA-fv2 2048^2 with active cells for 10000 steps.
@@ -47,11 +52,10 @@ HBM_PEAK_GBS = 8000.0
 # more) is needed only where a wet cell is within reach of the launch's
 # steps; the fused sweeps read it only there.
 MIN_BYTES_PER_CELL_SCIDDICAT = 16.0
-SWEEP_PROFILE = os.path.join(ROOT, "profiles", "sciddicat_sweep_4096x4096.md")
-SWEEP2_PROFILE = os.path.join(ROOT, "profiles", "sciddicat_sweep2x_4096x4096.md")
+TILES_PROFILE = os.path.join(ROOT, "profiles", "sciddicat_tiles_4096x4096.md")
 
 
-def profiled_traffic(path=SWEEP_PROFILE):
+def profiled_traffic(path=TILES_PROFILE):
     """HBM bytes per launch of the sweep from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE, calibrated against a device copy);
     None when the summary is absent."""
@@ -81,9 +85,9 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024, batched=True):
     m.init()
     m.steps(warmup)          # the first step uploads the model
     if batched:
-        m.run_steps(4)       # the two-step sweep's tile shape is chosen on first use
+        m.run_steps(4)       # the sweep's shape and the tile list are set up on first use
     m.sync()
-    m.timing(True)
+    m.timing(2)              # one HIP event pair on the model's stream around the timed steps
     t0 = time.perf_counter()
     if batched:
         m.run_steps(steps)
@@ -92,7 +96,7 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024, batched=True):
     m.sync()
     wall = time.perf_counter() - t0
     ktime_ms, launches = m.kernel_time()
-    m.timing(False)
+    m.timing(0)
     st = m.stats()
     _, hh, _ = m.read()
     mass_err = abs(hh.sum() - h.sum()) / h.sum()
@@ -102,8 +106,8 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024, batched=True):
     return {
         "wall_s": wall, "gcups": cells * steps / wall / 1e9, "kernel_ms_avg": kavg, "kernel_launches": launches,
         # the bytes one launch must move at least (h read once and written
-        # once: 16 B per cell, whether the launch advances one step or two)
-        # over the average launch time
+        # once: 16 B per cell, whatever its steps) over the average launch
+        # time (the timed region's GPU time over its launches)
         "achieved_gbs": MIN_BYTES_PER_CELL_SCIDDICAT * cells / (kavg * 1e-3) / 1e9,
         "steps_per_launch": steps / max(launches, 1),
         "launches_per_step": st["launches"] / max(st["steps"], 1), "mass_rel_err": mass_err,
@@ -214,6 +218,40 @@ def bench_active(n=1024, steps=4000, seed=7, cpu_steps=400):
     return out
 
 
+def bench_wet(n=4096, steps=100, seed=2024):
+    """SciddicaT 4096^2 full grid with debris on every cell (h ~ U(0.5, 5) m,
+    the headline's DEM): the minimisation runs everywhere, the tile mode
+    declines and the engine takes its strip sweeps (two steps per launch)."""
+    import native
+    from opencal_amd import synth
+    from opencal_amd.models import SciddicaT
+    z, _, _ = synth.sciddicat_inputs(n, n, seed)
+    h = 0.5 + 4.5 * synth.uniform(seed + 1, n * n).reshape(n, n)
+    m = SciddicaT(z, h, 0.5, 0.001, steps + 8, active=False)
+    m.init()
+    m.steps(1)
+    m.run_steps(4)          # the sweep's shape is chosen on first use
+    m.sync()
+    m.timing(True)
+    t0 = time.perf_counter()
+    m.run_steps(steps)
+    m.sync()
+    wall = time.perf_counter() - t0
+    ktime_ms, launches = m.kernel_time()
+    m.timing(False)
+    _, gh, _ = m.read()
+    m.close()
+    _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, steps + 5, False)
+    wet = float((oh > 0.001).mean())
+    return {"workload": f"sciddicaT_plain_wet_{n}x{n}_x{steps}", "gcups": n * n * steps / wall / 1e9,
+            "ms_per_step": wall / steps * 1e3, "launches_per_step": launches / steps,
+            "kernel_us_per_launch": ktime_ms / max(launches, 1) * 1e3,
+            "achieved_gbs_per_launch": MIN_BYTES_PER_CELL_SCIDDICAT * n * n / (ktime_ms / max(launches, 1) * 1e-3) / 1e9,
+            "wet_fraction_at_end": wet, "bit_exact_vs_oracle": bool(np.array_equal(gh, oh)),
+            "note": "wall time with per-launch HIP events; achieved = 16 B per cell over the average launch "
+                    "(each launch reads z, h and writes h for two steps: 24 B per cell)"}
+
+
 def bench_life(n=512, steps=1000, seed=1):
     import native
     from opencal_amd import synth
@@ -268,7 +306,7 @@ def bench_batch(n_inst=256, steps=2000, seed=4, sequential=True):
         return {"workload": f"sciddicaT_batch_{n_inst}_instances_128..1024_x{steps}", "cells": cells,
                 "batched_gcups": cells * steps / (ms * 1e-3) / 1e9, "batched_s": ms * 1e-3}
     # the same instances one after another, each through calRun2D's step loop
-    # (calhipRunSteps2D: two steps per sweep, the headline's driver)
+    # (calhipRunSteps2D, the headline's driver)
     seq_s = 0.0
     exact = True
     for (z, h, pr, pe), got in zip(insts, outs):
@@ -301,7 +339,7 @@ def bench_batch(n_inst=256, steps=2000, seed=4, sequential=True):
                          f"one after another, 100 steps each incl. init, {cw:.1f} s")
     return {"workload": f"sciddicaT_batch_{n_inst}_instances_128..1024_x{steps}", "cells": cells,
             "batched_gcups": bat_gcups, "sequential_gcups": seq_gcups, "speedup": bat_gcups / seq_gcups,
-            "sequential_driver": "each instance through calhipRunSteps2D (calRun2D's loop, two steps per sweep)",
+            "sequential_driver": "each instance through calhipRunSteps2D (calRun2D's loop, the headline's driver)",
             "batched_s": ms * 1e-3, "sequential_s": seq_s, "each_instance_equals_its_single_run": exact,
             "cpu_baseline": cpu}
 
@@ -322,7 +360,7 @@ def bench_adjoint(n=1024, steps=2000, ck=45, seed=5):
     torch.cuda.synchronize()
     fwd_s = time.perf_counter() - t0
     # the best plain run of the same forward: the C API model through
-    # calRun2D's loop (two steps per sweep)
+    # calRun2D's loop (the headline's driver)
     from opencal_amd.models import SciddicaT
     h_src = np.zeros_like(z_np)
     h_src[r0:r0 + p, c0:c0 + p] = h_np[r0:r0 + p, c0:c0 + p]
@@ -422,7 +460,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
-    ap.add_argument("--only", choices=["active", "life", "batch", "adjoint", "sciara"],
+    ap.add_argument("--only", choices=["active", "wet", "life", "batch", "adjoint", "sciara"],
                     help="run one secondary workload alone and print its result (for profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="with --only batch: the batched run alone, without the sequential and CPU legs")
@@ -434,8 +472,8 @@ def main():
 
     import torch
     assert torch.cuda.is_available(), "needs a GPU"
-    workloads = {"active": bench_active, "life": bench_life, "batch": bench_batch, "adjoint": bench_adjoint,
-                 "sciara": bench_sciara}
+    workloads = {"active": bench_active, "wet": bench_wet, "life": bench_life, "batch": bench_batch,
+                 "adjoint": bench_adjoint, "sciara": bench_sciara}
     if args.only:
         fn = workloads[args.only]
         if args.only == "batch" and args.batched_only:
@@ -463,29 +501,32 @@ def main():
         "config": {"workload": f"sciddicaT_plain_{args.size}x{args.size}_fp64", "rows": args.size,
                    "cols": args.size, "p_r": 0.5, "p_epsilon": 0.001, "optimization": "CAL_NO_OPT",
                    "driver": "calRun2D's step loop (calhipRunSteps2D), steering = the reference flow reset "
-                             "with its device twin: two steps per sweep"},
+                             "with its device twin: up to eight steps per launch"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(SWEEP2_PROFILE) if args.size == 4096 else None,
-                     "traffic_source": "profiles/sciddicat_sweep2x_4096x4096.md (rocprofv3 FETCH_SIZE x 2.0 + "
-                                       "WRITE_SIZE, per two-step launch)",
-                     "kernel": "step_fused_2x", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
+                     "traffic": profiled_traffic() if args.size == 4096 else None,
+                     "traffic_source": "profiles/sciddicat_tiles_4096x4096.md (rocprofv3 FETCH_SIZE x 2.0 + "
+                                       "WRITE_SIZE, per 8-step launch)",
+                     "kernel": "plain_tiles<8>", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
                      "steps_per_launch": round(r["steps_per_launch"], 4),
                      "algorithmic_bytes_per_launch": int(MIN_BYTES_PER_CELL_SCIDDICAT * args.size * args.size),
                      "note": "achieved = the bytes one launch must move (h read once and written once: 16 B "
-                             "per cell; the launch advances two steps and keeps the intermediate level in "
-                             "registers; z, 8 B more, is read only for strip chunks with a wet cell within two "
-                             "cells) / the launch's average duration from HIP events on the model's stream; "
+                             "per cell, whatever its steps; the launch advances 8 steps, the tiles within "
+                             "reach of a wet cell in LDS, reading their z and a halo, every other cell by a "
+                             "copy) / the launch's average duration (HIP events on the model's stream around "
+                             "the timed steps, over their launches); "
                              "traffic = measured HBM bytes per launch (rocprofv3 counters), traffic_gbs / "
                              "traffic_frac over the same time"},
         "per_step_driver": {
-            "driver": "calRunCAStep2D per step (one sweep per step)", "gcups": round(r1["gcups"], 3),
+            "driver": "calRunCAStep2D per step (the engine groups eight steps to a launch)",
+            "gcups": round(r1["gcups"], 3),
             "ms_per_step": round(r1["wall_s"] / args.steps * 1e3, 4),
             "roofline": {"bound": "hbm", "achieved": round(r1["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(r1["achieved_gbs"] / HBM_PEAK_GBS, 4),
                          "traffic": profiled_traffic() if args.size == 4096 else None,
-                         "traffic_source": "profiles/sciddicat_sweep_4096x4096.md",
-                         "kernel": "step_fused_al", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2),
+                         "traffic_source": "profiles/sciddicat_tiles_4096x4096.md",
+                         "kernel": "plain_tiles<8>", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2),
+                         "steps_per_launch": round(r1["steps_per_launch"], 4),
                          "algorithmic_bytes_per_launch": int(MIN_BYTES_PER_CELL_SCIDDICAT * args.size * args.size)}},
     }
     # the measured HBM bytes per launch over the same kernel time: the rate the

@@ -61,9 +61,11 @@ struct calhipStats {
 };
 int calhipGetStats2D(struct CALModel2D* ca2D, struct calhipStats* out);
 
-/* Kernel timing with HIP events recorded on the model's own stream around
- * each launch of the program's main sweep kernel.  enable=1 starts, 0 stops;
- * calhipGetKernelTime2D returns the accumulated milliseconds and launches. */
+/* Kernel timing with HIP events recorded on the model's own stream: enable=1
+ * around each launch of the program's main sweep kernel, enable=2 once when
+ * timing starts and once at each query (the whole region, launches back to
+ * back, over the engine's launches in it); 0 stops.  calhipGetKernelTime2D
+ * returns the milliseconds and the launches. */
 int calhipEnableKernelTiming2D(struct CALModel2D* ca2D, int enable);
 int calhipGetKernelTime2D(struct CALModel2D* ca2D, double* total_ms, long long* launches);
 

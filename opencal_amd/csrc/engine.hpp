@@ -138,6 +138,10 @@ struct Timer {
 	double total_ms = 0.0;
 	long long launches = 0;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+	// region mode: one event when timing starts, one at each query
+	bool region = false;
+	hipEvent_t region_start = nullptr;
+	long long region_launches0 = 0;
 };
 
 struct ModelX {

@@ -368,10 +368,21 @@ extern "C" int calhipEnableKernelTiming2D(struct CALModel2D* ca2D, int enable)
 	ModelX* m = X(ca2D);
 	if (m->prog && m->prog->flush(m)) return -1;
 	if (!enable) timer_collect(m);
-	m->timer.enabled = enable != 0;
+	if (m->timer.region_start) {
+		hipEventDestroy(m->timer.region_start);
+		m->timer.region_start = nullptr;
+	}
+	m->timer.region = enable == 2;
+	m->timer.enabled = enable == 1;
 	if (enable) {
 		m->timer.total_ms = 0.0;
 		m->timer.launches = 0;
+	}
+	if (m->timer.region) {
+		if (ensure_stream(m) || CALHIP_CHECK(hipEventCreate(&m->timer.region_start)) ||
+		    CALHIP_CHECK(hipEventRecord(m->timer.region_start, m->stream)))
+			return -1;
+		m->timer.region_launches0 = m->stats.launches;
 	}
 	return 0;
 }
@@ -380,6 +391,18 @@ extern "C" int calhipGetKernelTime2D(struct CALModel2D* ca2D, double* total_ms, 
 {
 	ModelX* m = X(ca2D);
 	if (m->prog && m->prog->flush(m)) return -1;
+	if (m->timer.region && m->timer.region_start) {
+		hipEvent_t end;
+		float ms = 0.f;
+		if (CALHIP_CHECK(hipEventCreate(&end))) return -1;
+		const int rc = CALHIP_CHECK(hipEventRecord(end, m->stream)) || CALHIP_CHECK(hipEventSynchronize(end)) ||
+		               CALHIP_CHECK(hipEventElapsedTime(&ms, m->timer.region_start, end));
+		hipEventDestroy(end);
+		if (rc) return -1;
+		*total_ms = ms;
+		*launches = m->stats.launches - m->timer.region_launches0;
+		return 0;
+	}
 	timer_collect(m);
 	*total_ms = m->timer.total_ms;
 	*launches = m->timer.launches;

@@ -50,8 +50,8 @@ class SciddicaT:
         check(self.E.sciddicat_steps(self.handle, int(n)), "sciddicat_steps")
 
     def run_steps(self, n):
-        """n steps of calRun2D's loop (calhipRunSteps2D): the full-grid model
-        with its steering twin advances two steps per sweep."""
+        """n steps of calRun2D's loop (calhipRunSteps2D): the model with its
+        steering twin advances several steps per launch."""
         check(self.E.sciddicat_run_steps(self.handle, int(n)), "sciddicat_run_steps")
 
     def clear_stop(self):
@@ -84,6 +84,8 @@ class SciddicaT:
         return stats(self.model)
 
     def timing(self, enable):
+        """1: events around each sweep launch; 2: one event pair around the
+        whole region (launches back to back); 0: off."""
         lib().calhipEnableKernelTiming2D(self.model, int(enable))
 
     def kernel_time(self):
