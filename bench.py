@@ -28,7 +28,10 @@ The same JSON line carries
                 against the same instances run
                 one after another, and the adjoint gradient / forward time
                 ratio at 1024^2 x 2000 with checkpoints every 45 steps, and
-                SCIARA-fv2 2048^2 with active cells for 10000 steps.
+                SCIARA-fv2 2048^2 with active cells for 10000 steps; each
+                runs in a child process (`bench.py --only NAME`) that this
+                run waits for, so that a failure there is reported in its
+                record and leaves the headline standing.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -37,6 +40,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -451,6 +455,30 @@ def bench_sciara(n=2048, seed=3, cpu_steps=200):
     return out
 
 
+def run_workload_child(name, timeout_s=300):
+    """One secondary workload in a child process (`bench.py --only name`),
+    waited for before returning: a crash or a time-out there becomes an error
+    record in this line instead of ending the run, and the child's last
+    stderr lines (with a Python traceback on a crash) come with it."""
+    cmd = [sys.executable, "-X", "faulthandler", os.path.abspath(__file__), "--only", name]
+    try:
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired as e:   # run() kills and reaps the child
+        tail = (e.stderr or b"")
+        tail = tail.decode(errors="replace") if isinstance(tail, bytes) else tail
+        return {"error": f"time-out after {timeout_s} s", "stderr_tail": tail.splitlines()[-20:]}
+    for line in reversed(p.stdout.splitlines()):
+        if line.startswith("{"):
+            try:
+                rec = json.loads(line).get(name)
+            except ValueError:
+                break
+            if p.returncode == 0 and rec is not None:
+                return rec
+            break
+    return {"error": f"exit status {p.returncode}", "stderr_tail": p.stderr.splitlines()[-20:]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -543,13 +571,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline_sciddicat(args.size, args.cpu_steps)
     if not args.quick:
         wl = {}
-        for name, fn in workloads.items():
+        for name in workloads:
             log(f"{name} ...")
             t0 = time.perf_counter()
-            try:
-                wl[name] = fn()
-            except Exception as e:   # report, do not hide: the line still carries the error
-                wl[name] = {"error": repr(e)}
+            wl[name] = run_workload_child(name)
             log(f"{name}: {json.dumps(wl[name])} ({time.perf_counter() - t0:.1f} s)")
         out["workloads"] = wl
     print(json.dumps(out), flush=True)

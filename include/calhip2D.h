@@ -58,6 +58,8 @@ struct calhipStats {
 	long long downloads;       /* device -> host model transfers */
 	long long rebuilds;        /* active-set compactions */
 	long long materialisations;/* transient substates written out on demand */
+	long long reruns;          /* steps run again through another path (a fused step that met a case it
+	                              leaves to the general kernels) */
 };
 int calhipGetStats2D(struct CALModel2D* ca2D, struct calhipStats* out);
 

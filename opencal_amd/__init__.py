@@ -56,7 +56,8 @@ def examples() -> C.CDLL:
 
 class CalhipStats(C.Structure):
     _fields_ = [("steps", C.c_longlong), ("launches", C.c_longlong), ("uploads", C.c_longlong),
-                ("downloads", C.c_longlong), ("rebuilds", C.c_longlong), ("materialisations", C.c_longlong)]
+                ("downloads", C.c_longlong), ("rebuilds", C.c_longlong), ("materialisations", C.c_longlong),
+                ("reruns", C.c_longlong)]
 
 
 def _declare(L):

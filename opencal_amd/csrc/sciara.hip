@@ -376,6 +376,269 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_finish(double* __restrict__ h, do
 			for (int q = 0; q < NF; ++q) f.f[q][c] = 0.0;
 }
 
+// ---- two launches per transition (flows, width) ----
+//
+// A transition followed by the steering twin, in active mode, as two
+// launches: sc_flows_f and sc_width_f.  Between two such transitions the
+// engine keeps the state in a form the next sc_flows_f completes:
+//   * the list P of the cells the last width process visited (the set
+//     before it plus its receivers), count cnt[t % 3];
+//   * flags: 1 for the cells of the set after the removal, 0 elsewhere;
+//   * Tn: the new temperature of P's cells (T not yet copied back);
+//   * the flows of the set's cells not yet reset by the steering.
+// sc_flows_f copies Tn back into T over P, lists the set's cells (the
+// survivors in P) and its receivers in the next list, applies the vents'
+// emission where it reads h and T (a vent's cell, when it is in the set,
+// reads as h + e and PTvent, the values sc_vents would have stored), writes
+// all eight flows of every cell of the set (the steering's reset folded
+// in) and marks its receivers with flag bit 1.  sc_width_f then runs the
+// width, temperature and removal processes over the new list, applies the
+// steering's border reset to the cells it visits (the others hold h = T = 0
+// already), writes flags 0/1, counts the survivors, and marks the vents
+// whose cell stayed in the set for the next transition's emission.  When
+// the set is empty OpenCAL's processes cover the whole grid; sc_flows_f
+// then records the transition number in *stall and this and every later
+// fused launch does nothing: the host finishes the last completed
+// transition and runs the stalled ones again through the four launches.
+struct FusedCtl {
+	const int* P;          // the list of the previous transition's width
+	int* Lnew;             // the list of this transition
+	int* cnt;              // [3] list lengths by transition mod 3
+	int* surv;             // [3] survivors by transition mod 3
+	int* stall;            // transition at which the set was found empty (0: none)
+	int* vent_act;         // [VENTS_PER_LAUNCH] transition of each vent's last emission
+	const int* count0;     // first fused transition: P is the set list, count0[0] entries
+	int t;                 // transition number (>= 1)
+	int entry;             // first fused transition: T is current, P = the set
+};
+
+__device__ __forceinline__ int vent_index(const VentArgs& v, int c)
+{
+	int r = -1;
+#pragma unroll
+	for (int k = 0; k < VENTS_PER_LAUNCH; ++k)
+		if (k < v.n && v.cell[k] == c) r = k;
+	return r;
+}
+
+__global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict__ z, const double* __restrict__ h,
+                                                       double* __restrict__ T, const double* __restrict__ Tn, Flows8 f,
+                                                       uint8_t* flags, FusedCtl F, VentArgs v, int rows, int cols,
+                                                       SciaraParams P)
+{
+	const int tp = (F.t + 2) % 3, tc = F.t % 3, tn = (F.t + 1) % 3;
+	if (*F.stall) return;
+	const int np = F.entry ? F.count0[0] : F.cnt[tp];
+	const int sp = F.entry ? F.count0[0] : F.surv[tp];
+	if (sp == 0) {
+		if (blockIdx.x == 0 && threadIdx.x == 0) *F.stall = F.t;
+		return;
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		F.cnt[tn] = 0;    // the next transition's list length
+		F.surv[tc] = 0;   // this transition's survivors (sc_width_f)
+	}
+	const int lane = (int)(threadIdx.x & 63);
+	const int g0 = lane & ~7;
+	const int k = (lane & 7) + 1;
+	const bool writer = (lane & 7) == 0;
+	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
+	const size_t idx0 = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3);
+	// a wave's groups agree on their trip count: the loop runs to the end of
+	// the wave's last group so that the survivors' ballot is wave-uniform
+	for (size_t base = idx0 & ~(size_t)7; base < (size_t)np; base += stride) {
+		const size_t idx = base + ((threadIdx.x >> 3) & 7);
+		const bool valid = idx < (size_t)np;
+		const int c = valid ? F.P[idx] : 0;
+		const unsigned fl = valid ? flags[c] : 0u;
+		double t = 0.0;
+		if (valid) {
+			t = F.entry ? T[c] : Tn[c];
+			if (!F.entry && writer) T[c] = t;
+		}
+		const bool member = valid && (fl & 1u);
+		// survivors to the new list: one atomic per wave
+		const unsigned long long wm = __ballot(member && writer);
+		int slot = 0;
+		if (wm) {
+			int basepos = 0;
+			if (lane == 0) basepos = atomicAdd(F.cnt + tc, __popcll(wm));
+			basepos = __shfl(basepos, 0, 64);
+			slot = basepos + __popcll(wm & ((1ull << lane) - 1ull));
+		}
+		if (member && writer) F.Lnew[slot] = c;
+		if (!member) continue;
+		const int vk = vent_index(v, c);
+		double h0 = h[c];
+		if (vk >= 0) {
+			h0 = h0 + v.e[vk];
+			t = P.PTvent;
+			if (writer) F.vent_act[vk] = F.t;
+		}
+		if (!(h0 > 0)) {
+			f.f[k - 1][c] = 0.0;
+			continue;
+		}
+		const int i = c / cols, j = c - (c / cols) * cols;
+		const double Pr = cr_pow10(P.a + P.b * t);
+		const double hc = cr_pow10(P.c + P.d * t);
+		const double sz0 = z[c];
+		const int nbk = nb_flat(i, j, k, rows, cols);
+		double hk = 0.0, zr = 0.0;
+		if (nbk >= 0) {
+			hk = h[nbk];
+			zr = z[nbk];
+			const int vn = vent_index(v, nbk);
+			if (vn >= 0 && (flags[nbk] & 1u)) hk = hk + v.e[vn];
+		}
+		const double zk = k < 5 ? zr : sz0 - (sz0 - zr) / P.rad2;
+		const double lvl0 = sz0 + h0;
+		const double lk = zk + hk;
+		const bool down = lvl0 > lk;
+		const double Hk = down ? lk : 0.0;
+		const double hcck = hc * cr_cos(cr_atan((lvl0 - lk) / P.Pc));
+		double H[9];
+		H[0] = sz0;
+#pragma unroll
+		for (int q = 1; q < 9; ++q) H[q] = __shfl(Hk, g0 + q - 1, 64);
+		unsigned keep = 1u | ((unsigned)((__ballot((int)down) >> g0) & 0xffull) << 1);
+		double avg;
+		bool loop;
+		do {
+			loop = false;
+			avg = h0;
+			int counter = 0;
+#pragma unroll
+			for (int q = 0; q < 9; ++q)
+				if ((keep >> q) & 1u) {
+					avg += H[q];
+					counter++;
+				}
+			if (counter != 0) avg = avg / (double)counter;
+#pragma unroll
+			for (int q = 0; q < 9; ++q)
+				if (((keep >> q) & 1u) && avg <= H[q]) {
+					keep &= ~(1u << q);
+					loop = true;
+				}
+		} while (loop);
+		double fk = 0.0;
+		if (((keep >> k) & 1u) && h0 > hcck) fk = Pr * (avg - Hk);
+		// the steering reset every flow of the set's cells; the process
+		// writes the positive ones (Sciara.cpp:95-101)
+		f.f[k - 1][c] = fk > 0 ? fk : 0.0;
+		if (fk > 0 && nbk >= 0) {
+			unsigned* w = reinterpret_cast<unsigned*>(flags + (nbk & ~3));
+			const int sh = (nbk & 3) * 8;
+			const unsigned old = atomicOr(w, 2u << sh);
+			if (((old >> sh) & 3u) == 0u) F.Lnew[atomicAdd(F.cnt + tc, 1)] = nbk;
+		}
+	}
+}
+
+__global__ __launch_bounds__(SC_BLOCK) void sc_width_f(double* __restrict__ z, double* __restrict__ h,
+                                                       const double* __restrict__ T, double* __restrict__ Tn,
+                                                       Flows8 f, const uint8_t* __restrict__ Mb,
+                                                       double* __restrict__ Msl, uint8_t* __restrict__ flags,
+                                                       FusedCtl F, VentArgs v, int rows, int cols, SciaraParams P)
+{
+	constexpr int opposite[NF] = {3, 2, 1, 0, 6, 7, 4, 5};
+	if (*F.stall) return;
+	const int tc = F.t % 3;
+	const int cnt = F.cnt[tc];
+	const int lane = (int)(threadIdx.x & 63);
+	const int g0 = lane & ~7;
+	const int k = (lane & 7) + 1;
+	const bool writer = (lane & 7) == 0;
+	const double* fin = f.f[opposite[0]];
+	const double* fout = f.f[0];
+#pragma unroll
+	for (int q = 1; q < NF; ++q)
+		if (k - 1 == q) {
+			fin = f.f[opposite[q]];
+			fout = f.f[q];
+		}
+	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
+	const size_t idx0 = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3);
+	for (size_t base = idx0 & ~(size_t)7; base < (size_t)cnt; base += stride) {
+		const size_t idx = base + ((threadIdx.x >> 3) & 7);
+		const bool valid = idx < (size_t)cnt;
+		bool surv = false;
+		if (valid) {
+			const int c = F.Lnew[idx];
+			const int i = c / cols, j = c - (c / cols) * cols;
+			double ih = h[c], it = T[c];
+			const int vk = vent_index(v, c);
+			if (vk >= 0 && F.vent_act[vk] == F.t) {
+				ih = ih + v.e[vk];
+				it = P.PTvent;
+			}
+			const int qk = nb_flat(i, j, k, rows, cols);
+			const double in_k = qk >= 0 ? fin[qk] : 0.0;
+			const double out_k = fout[c];
+			double nt_k = 0.0;
+			if (qk >= 0) {
+				nt_k = T[qk];
+				const int vn = vent_index(v, qk);
+				if (vn >= 0 && F.vent_act[vn] == F.t) nt_k = P.PTvent;
+			}
+			const double ht_k = in_k * nt_k;
+			double ht = 0.0, in_sum = 0.0, out_sum = 0.0;
+#pragma unroll
+			for (int m = 0; m < 8; ++m) {
+				ht += __shfl(ht_k, g0 + m, 64);
+				in_sum += __shfl(in_k, g0 + m, 64);
+				out_sum += __shfl(out_k, g0 + m, 64);
+			}
+			double h1 = ih;
+			h1 += in_sum - out_sum;
+			double t1 = it;
+			if (in_sum > 0 || out_sum > 0) {
+				double residual = ih;
+				residual -= out_sum;
+				t1 = (residual * it + ht) / (residual + in_sum);
+			}
+			double h2 = h1, t2 = t1;
+			const bool mb = Mb[c] != 0;
+			if (h1 > 0 && !mb) {
+				const double nT = t1;
+				const double num = (((3.0 * cr_cube(nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
+				const double den = ((P.Prho * P.Pcv) * h1) * P.Pac;
+				const double aus = 1.0 + num / den;
+				const double st = nT / cr_pow_third(aus);
+				t2 = st;
+				if (st <= P.PTsol) {
+					if (writer) {
+						z[c] = z[c] + h1;
+						Msl[c] = Msl[c] + h1;
+					}
+					h2 = 0.0;
+					t2 = P.PTsol;
+				}
+			}
+			surv = !(t2 <= P.PTsol && !mb);
+			if (writer) {
+				// the steering's border reset (Sciara.cpp:189-194) after the removal
+				h[c] = mb ? 0.0 : h2;
+				Tn[c] = mb ? 0.0 : t2;
+				flags[c] = surv ? 1 : 0;
+			}
+		}
+		const unsigned long long wm = __ballot(surv && writer);
+		if (lane == 0 && wm) atomicAdd(F.surv + tc, __popcll(wm));
+	}
+}
+
+// finishing the fused transitions: count[0] = the last list's length, the
+// other counters cleared (sc_finish then runs over that list)
+__global__ void sc_fused_counts(int* count, const int* cnt_slot)
+{
+	count[0] = *cnt_slot;
+	count[1] = 0;
+	count[2] = 0;
+	count[3] = 0;
+}
+
 // steering alone (the run loop called the twin separately from the transition)
 template <bool ACTIVE>
 __global__ __launch_bounds__(SC_BLOCK) void sc_steer(double* __restrict__ h, double* __restrict__ T, Flows8 f,
@@ -431,12 +694,21 @@ struct SciaraProgram : Program {
 	long long est = 0;
 	long long steps_done = 0;
 
+	// fused transitions (sc_flows_f / sc_width_f)
+	bool fz_on = false;
+	int fz_t = 1;                  // number of the next fused transition (from 1)
+	int* fz_buf = nullptr;         // cnt[3], surv[3], stall, vent_act[VENTS_PER_LAUNCH]
+	int* fz_lists[2] = {nullptr, nullptr};   // the list buffers (d_cells, alt at entry)
+	std::vector<double> fz_elapsed;           // elapsed time at the start of each fused transition
+	bool fz_replaying = false;                // running stalled transitions again (four launches)
+
 	~SciaraProgram() override
 	{
 		if (alt) (void)hipFree(alt);
 		if (d_mb) (void)hipFree(d_mb);
 		if (h_cnt) (void)hipHostFree(h_cnt);
 		if (ev_cnt) (void)hipEventDestroy(ev_cnt);
+		if (fz_buf) (void)hipFree(fz_buf);
 	}
 
 	const char* name() const override { return "SCIARA-fv2"; }
@@ -498,11 +770,14 @@ struct SciaraProgram : Program {
 			cnt_pending = false;
 		}
 		unordered = false;
+		fz_on = false;    // the upload replaced the state the fused transitions kept
+		fz_elapsed.clear();
 		return CALHIP_CHECK(hipStreamSynchronize(m->stream));
 	}
 
 	int sync_active(ModelX* m) override
 	{
+		if (fz_finish(m)) return -1;
 		if (!unordered) return 0;
 		// flags equal the set; rebuild the row-major list from them
 		if (k::rebuild_active(m)) return -1;
@@ -588,8 +863,148 @@ struct SciaraProgram : Program {
 		return clamp_grid((size_t)(est + est / 4 + 1024) * (size_t)lanes, 8, 8192);
 	}
 
+	// ---- fused transitions ----
+
+	bool fz_eligible(ModelX* m) const
+	{
+		// opt-in: a 2048 x 2048 run of this path ended in a host-side crash
+		// whose cause is not found yet; the four launches stay the default
+		const char* on = getenv("CALHIP_SCIARA_TWO_LAUNCH");
+		if (!active || fz_replaying || m->flags_dirty || !on || strcmp(on, "1") != 0) return false;
+		std::vector<std::pair<int, double>> vl;
+		vent_list(vl, m->pub.columns);
+		return vl.size() <= (size_t)VENTS_PER_LAUNCH;
+	}
+
+	FusedCtl fz_ctl(ModelX* m, int t) const
+	{
+		FusedCtl F;
+		const int s = t - 1;   // transitions since the entry
+		F.P = fz_lists[s % 2];
+		F.Lnew = fz_lists[(s + 1) % 2];
+		F.cnt = fz_buf;
+		F.surv = fz_buf + 3;
+		F.stall = fz_buf + 6;
+		F.vent_act = fz_buf + 7;
+		F.count0 = m->d_count;
+		F.t = t;
+		F.entry = t == 1 ? 1 : 0;
+		return F;
+	}
+
+	int fused_transition(ModelX* m)
+	{
+		if (check_entry(m)) return -1;
+		refresh_estimate();
+		if (!fz_on) {
+			const size_t bytes = sizeof(int) * (7 + VENTS_PER_LAUNCH);
+			if (!fz_buf && CALHIP_CHECK(hipMalloc(&fz_buf, bytes))) return -1;
+			if (CALHIP_CHECK(hipMemsetAsync(fz_buf, 0, bytes, m->stream))) return -1;
+			fz_lists[0] = m->d_cells;
+			fz_lists[1] = alt;
+			fz_t = 1;
+			fz_elapsed.clear();
+			fz_on = true;
+		}
+		const SciaraParams P = params();
+		const int rows = m->pub.rows, cols = m->pub.columns;
+		std::vector<std::pair<int, double>> vl;
+		vent_list(vl, cols);
+		VentArgs va;
+		va.n = (int)vl.size();
+		for (int k = 0; k < va.n; ++k) {
+			va.cell[k] = vl[k].first;
+			va.e[k] = vl[k].second;
+		}
+		fz_elapsed.push_back(*b.elapsed_time);
+		const FusedCtl F = fz_ctl(m, fz_t);
+		const int grid8 = list_grid(m, 8);
+		hipEvent_t ev;
+		timer_begin(m, &ev);
+		hipLaunchKernelGGL(sc_flows_f, dim3(grid8), dim3(SC_BLOCK), 0, m->stream, (const double*)z->d_cur,
+		                   (const double*)h->d_cur, (double*)T->d_cur, (const double*)T->d_nxt, flows(), m->d_flags,
+		                   F, va, rows, cols, P);
+		hipLaunchKernelGGL(sc_width_f, dim3(grid8), dim3(SC_BLOCK), 0, m->stream, (double*)z->d_cur,
+		                   (double*)h->d_cur, (const double*)T->d_cur, (double*)T->d_nxt, flows(),
+		                   (const uint8_t*)mb->d_cur, (double*)msl->d_cur, m->d_flags, F, va, rows, cols, P);
+		timer_end(m, ev);
+		m->stats.launches += 2;
+		if (CALHIP_CHECK(hipGetLastError())) return -1;
+		*b.elapsed_time += *b.Pclock;
+		if ((++steps_done & 15) == 0 && !cnt_pending) {
+			if (CALHIP_CHECK(hipMemcpyAsync(h_cnt, fz_buf + fz_t % 3, sizeof(int), hipMemcpyDeviceToHost,
+			                                m->stream)) ||
+			    CALHIP_CHECK(hipEventRecord(ev_cnt, m->stream)))
+				return -1;
+			cnt_pending = true;
+		}
+		fz_t++;
+		unordered = true;
+		T->nxt_stale = true;
+		return 0;
+	}
+
+	// Back to the four-launch representation: the last completed fused
+	// transition finished by sc_finish (temperatures copied back, the
+	// steering's flow reset, the set compacted into the list); transitions
+	// that found the set empty run again through the four launches.
+	int fz_finish(ModelX* m)
+	{
+		if (!fz_on) return 0;
+		fz_on = false;
+		int stall = 0;
+		if (CALHIP_CHECK(hipMemcpyAsync(&stall, fz_buf + 6, sizeof(int), hipMemcpyDeviceToHost, m->stream)) ||
+		    CALHIP_CHECK(hipStreamSynchronize(m->stream)))
+			return -1;
+		const int last = stall ? stall - 1 : fz_t - 1;   // last completed fused transition
+		if (last >= 1) {
+			int* done_list = fz_lists[last % 2];
+			int* other = fz_lists[(last + 1) % 2];
+			hipLaunchKernelGGL(sc_fused_counts, dim3(1), dim3(1), 0, m->stream, m->d_count, fz_buf + last % 3);
+			hipLaunchKernelGGL((sc_finish<true, true>), dim3(list_grid(m)), dim3(SC_BLOCK), 0, m->stream,
+			                   (double*)h->d_cur, (double*)T->d_cur, (const double*)T->d_nxt, flows(),
+			                   (const uint8_t*)mb->d_cur, (const uint8_t*)m->d_flags, done_list, other, m->d_count,
+			                   d_mb, n_mb, m->pub.rows, m->pub.columns);
+			m->stats.launches += 2;
+			if (CALHIP_CHECK(hipGetLastError())) return -1;
+			m->d_cells = other;
+			alt = done_list;
+			unordered = true;
+		} else {
+			m->d_cells = fz_lists[0];
+			alt = fz_lists[1];
+		}
+		if (stall) {
+			// the transitions from `stall` on left the state as it was: run
+			// them again, each with the elapsed time it started from
+			const int redo = fz_t - stall;
+			const double end = *b.elapsed_time;
+			*b.elapsed_time = fz_elapsed[(size_t)(stall - 1)];
+			fz_elapsed.clear();
+			fz_replaying = true;
+			m->stats.reruns += redo;
+			for (int r = 0; r < redo; ++r)
+				if (transition(m, true)) {
+					fz_replaying = false;
+					return -1;
+				}
+			fz_replaying = false;
+			if (*b.elapsed_time != end) {
+				set_error(12, "SCIARA fused transitions: elapsed time differs after the re-run");
+				return -1;
+			}
+		}
+		fz_elapsed.clear();
+		return 0;
+	}
+
+	int before_read(ModelX* m) override { return fz_finish(m); }
+	int before_modify(ModelX* m, SubX*) override { return fz_finish(m); }
+
 	int transition(ModelX* m, bool steer)
 	{
+		if (steer && fz_eligible(m)) return fused_transition(m);
+		if (fz_finish(m)) return -1;
 		if (check_entry(m)) return -1;
 		refresh_estimate();
 		const SciaraParams P = params();
@@ -671,6 +1086,7 @@ struct SciaraProgram : Program {
 	// Sciara.cpp:185-197 on the device
 	int steer(ModelX* m) override
 	{
+		if (fz_finish(m)) return -1;
 		const int rows = m->pub.rows, cols = m->pub.columns;
 		refresh_estimate();
 		const int grid = list_grid(m);

@@ -124,3 +124,110 @@ def test_sciara_properties_1024():
     assert (T[h > 0] > inp["params"][1]).all() and (T[h > 0] <= inp["params"][2]).all()
     assert np.allclose(dev["z"] - msl, inp["z0"], rtol=0, atol=1e-9)
     assert (msl > 0).sum() > 0
+
+
+# the two-launch transitions are opt-in (CALHIP_SCIARA_TWO_LAUNCH=1); the
+# four launches are the default
+TWO = {"CALHIP_SCIARA_TWO_LAUNCH": "1"}
+FOUR = {"CALHIP_SCIARA_TWO_LAUNCH": "0"}
+
+
+def _run_steps(inp, steps, env, empty_set=False, chunk=None, keep=()):
+    import ctypes as C
+    import os
+    from opencal_amd import lib
+    from opencal_amd.models import Sciara
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    m = Sciara(inp, steps + 10, True)
+    try:
+        m.init()
+        if empty_set:
+            L = lib()
+            L.calRemoveActiveCell2D.argtypes = [C.c_void_p, C.c_int, C.c_int]
+            for i in range(m.rows):
+                for j in range(m.cols):
+                    L.calRemoveActiveCell2D(m.model, i, j)
+            L.calAddActiveCell2D.argtypes = [C.c_void_p, C.c_int, C.c_int]
+            for i, j in keep:
+                L.calAddActiveCell2D(m.model, int(i), int(j))
+        ran = 0
+        for k in range(0, steps, chunk or steps):
+            ran += m.steps(min(chunk or steps, steps - k))
+            out = m.read()          # the read finishes the fused transitions (and re-runs stalled ones)
+        st = m.stats()
+        out["ran"] = ran
+        out["reruns"] = st["reruns"]
+        return out, st["launches"]
+    finally:
+        m.close()
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def test_sciara_two_launch_transitions():
+    """calRunCAStep2D with the steering twin runs a transition as two launches
+    (flows with the emission and the last transition's finish folded in,
+    then width with removal and the border reset); the state equals the
+    four-launch path's bit for bit, and the crmath oracle's."""
+    inp = _inputs(96, 128, 11, [[30, 50], [30, 50], [44, 90]], steps_per_rate=60, n_rates=4)
+    steps = 200
+    a, la = _run_steps(inp, steps, TWO)
+    b, lb = _run_steps(inp, steps, FOUR)
+    ora = native.oracle_sciara(inp, steps, True, crmath=True)
+    print(f"launches two-launch {la}, four-launch {lb}", flush=True)
+    for k in ("z", "h", "T", "Msl"):
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], ora[k]), k
+    assert a["active"] == b["active"] == ora["active"]
+    assert a["elapsed"] == b["elapsed"]
+    assert la <= 2 * steps + 8 < lb
+
+
+def test_sciara_two_launch_set_empties():
+    """The host empties the set before the first step and the lava solidifies
+    fast (PTsol 5 K below the vent temperature, emission in the first
+    interval only), so the set goes empty again and again: OpenCAL then runs
+    every process over the whole grid.  The two-launch transitions stop at
+    an empty set and the engine runs those transitions again through the
+    four launches; the state equals the four-launch path's bit for bit."""
+    inp = _inputs(64, 96, 7, [[14, 40]], steps_per_rate=20, n_rates=5)
+    rates = inp["rates"].copy()
+    rates[:, 1:] = 0.0
+    inp["rates"] = np.ascontiguousarray(rates)
+    prm = inp["params"].copy()
+    prm[1] = 1359.0
+    inp["params"] = prm
+    for steps in (15, 40, 100):
+        print(f"case sciara empty set, {steps} steps", flush=True)
+        a, la = _run_steps(inp, steps, TWO, empty_set=True)
+        b, lb = _run_steps(inp, steps, FOUR, empty_set=True)
+        print(f"launches two-launch {la}, four-launch {lb}, active {a['active']}", flush=True)
+        for k in ("z", "h", "T", "Msl"):
+            assert np.array_equal(a[k], b[k]), f"steps={steps} {k}"
+        assert a["active"] == b["active"] and a["elapsed"] == b["elapsed"] and a["ran"] == b["ran"]
+        if steps == 100:
+            assert a["active"] == 0
+            assert a["reruns"] > 40      # the transitions after the set emptied ran again in four launches
+    # a set of the vent cell alone, with a thin emission that never flows
+    # (h stays below the critical thickness): the vent stays in the set while
+    # it emits and leaves it when its lava cools below PTsol = 1300 K some
+    # steps after the emission stops, part-way through the fused run
+    inp2 = synth.sciara_inputs(64, 96, 7, steps_per_rate=20, n_rates=5, vol_lo=0.5, vol_hi=1.0)
+    inp2["vents"] = np.ascontiguousarray(np.array([[14, 40]], dtype=np.int32))
+    r2 = inp2["rates"][:1].copy()
+    r2[:, 1:] = 0.0
+    inp2["rates"] = np.ascontiguousarray(r2)
+    p2 = inp2["params"].copy()
+    p2[1] = 1300.0
+    inp2["params"] = p2
+    a, _ = _run_steps(inp2, 100, TWO, empty_set=True, keep=[(14, 40)])
+    b, _ = _run_steps(inp2, 100, FOUR, empty_set=True, keep=[(14, 40)])
+    print(f"vent alone: reruns {a['reruns']}, active {a['active']}", flush=True)
+    for k in ("z", "h", "T", "Msl"):
+        assert np.array_equal(a[k], b[k]), f"vent alone {k}"
+    assert a["active"] == b["active"] == 0 and a["elapsed"] == b["elapsed"]
+    assert 0 < a["reruns"] < 90
