@@ -407,7 +407,7 @@ def bench_adjoint(n=1024, steps=2000, ck=45, seed=5):
             "fd_dL_dpr": fd, "fd_rel_err": abs(g - fd) / max(abs(fd), 1e-30)}
 
 
-def bench_sciara(n=2048, seed=3, cpu_steps=200):
+def bench_sciara(n=2048, seed=3, cpu_steps=200, cpu=True):
     """SCIARA-fv2 lava flow (config: 2048^2, Moore, active cells, 3 vents, 10
     emission intervals of 1000 steps; the model's stop condition ends the run
     after 10000 steps), stepped through calRunCAStep2D with the steering twin,
@@ -441,18 +441,55 @@ def bench_sciara(n=2048, seed=3, cpu_steps=200):
     # first cpu_steps steps of the same run, else the oracle restatement; the
     # cost per step is dominated by the full-grid active-set rebuilds and
     # barely depends on the lava extent
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    # (the reference build runs in a subprocess: its OpenMP run has ended in
+    # a segmentation fault on some hosts, which then costs only this leg)
+    if not cpu:
+        return out
     kind = "reference" if native.ref("sciara") is not None else "port"
-    t0 = time.perf_counter()
+    cw, note = None, ""
     if kind == "reference":
-        native.ref_sciara(inp, cpu_steps, True)
-    else:
+        cw, err = ref_leg_child(["sciara", str(n), str(seed), str(cpu_steps)])
+        if cw is None:
+            kind, note = "port", f"; the reference build's run failed ({err}), the oracle was timed instead"
+    if cw is None:
+        t0 = time.perf_counter()
         native.oracle_sciara(inp, cpu_steps, True, crmath=False)
-    cw = time.perf_counter() - t0
+        cw = time.perf_counter() - t0
     out["cpu_baseline"] = cpu_record(cw / cpu_steps * 1e3, "ms/step", kind,
                                      f"{'reference OpenCAL-OMP Sciara.cpp' if kind == 'reference' else 'oracle'} "
-                                     f"{n}x{n} active cells, first {cpu_steps} steps incl. init, {cw:.1f} s")
+                                     f"{n}x{n} active cells, first {cpu_steps} steps incl. init, {cw:.1f} s"
+                                     f"{note}")
+    if kind == "port":
+        out["cpu_baseline"]["cores"] = 1
     return out
+
+
+def ref_leg(name, n, seed, steps):
+    """The CPU leg of a workload on the reference build alone (no GPU use);
+    returns the seconds it took."""
+    import native
+    from opencal_amd import synth
+    if name != "sciara":
+        raise ValueError(name)
+    inp = synth.sciara_inputs(n, n, seed)
+    t0 = time.perf_counter()
+    native.ref_sciara(inp, steps, True)
+    return time.perf_counter() - t0
+
+
+def ref_leg_child(args, timeout_s=240):
+    """ref_leg in a child process (`bench.py --ref-leg NAME N SEED STEPS`),
+    waited for: (seconds, None), or (None, what went wrong)."""
+    cmd = [sys.executable, "-X", "faulthandler", os.path.abspath(__file__), "--ref-leg", *args]
+    try:
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:   # run() kills and reaps the child
+        return None, f"time-out after {timeout_s} s"
+    if p.returncode == 0:
+        for line in reversed(p.stdout.splitlines()):
+            if line.startswith("{"):
+                return float(json.loads(line)["seconds"]), None
+    return None, f"exit status {p.returncode}"
 
 
 def run_workload_child(name, timeout_s=300):
@@ -492,7 +529,13 @@ def main():
                     help="run one secondary workload alone and print its result (for profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="with --only batch: the batched run alone, without the sequential and CPU legs")
+    ap.add_argument("--ref-leg", nargs=4, metavar=("NAME", "N", "SEED", "STEPS"),
+                    help="time a workload's CPU leg on the reference build alone (used as a child process)")
     args = ap.parse_args()
+    if args.ref_leg:
+        name, n, seed, steps = args.ref_leg
+        print(json.dumps({"seconds": ref_leg(name, int(n), int(seed), int(steps))}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the engine runs on one MI355X; "
               "how the path would shard is a design note)", file=sys.stderr)
@@ -504,6 +547,8 @@ def main():
                  "adjoint": bench_adjoint, "sciara": bench_sciara}
     if args.only:
         fn = workloads[args.only]
+        if args.only == "sciara" and args.no_cpu:
+            fn = lambda: bench_sciara(cpu=False)   # noqa: E731
         if args.only == "batch" and args.batched_only:
             fn = lambda: bench_batch(sequential=False)   # noqa: E731
         print(json.dumps({args.only: fn()}), flush=True)

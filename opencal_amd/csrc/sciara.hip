@@ -382,34 +382,50 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_finish(double* __restrict__ h, do
 // launches: sc_flows_f and sc_width_f.  Between two such transitions the
 // engine keeps the state in a form the next sc_flows_f completes:
 //   * the list P of the cells the last width process visited (the set
-//     before it plus its receivers), count cnt[t % 3];
+//     before it plus its receivers, less the set-aside border cells below),
+//     count cnt[t % 3];
 //   * flags: 1 for the cells of the set after the removal, 0 elsewhere;
 //   * Tn: the new temperature of P's cells (T not yet copied back);
 //   * the flows of the set's cells not yet reset by the steering.
 // sc_flows_f copies Tn back into T over P, lists the set's cells (the
-// survivors in P) and its receivers in the next list, applies the vents'
+// members of P) and its receivers in the next list, applies the vents'
 // emission where it reads h and T (a vent's cell, when it is in the set,
 // reads as h + e and PTvent, the values sc_vents would have stored), writes
 // all eight flows of every cell of the set (the steering's reset folded
-// in) and marks its receivers with flag bit 1.  sc_width_f then runs the
-// width, temperature and removal processes over the new list, applies the
-// steering's border reset to the cells it visits (the others hold h = T = 0
-// already), writes flags 0/1, counts the survivors, and marks the vents
-// whose cell stayed in the set for the next transition's emission.  When
-// the set is empty OpenCAL's processes cover the whole grid; sc_flows_f
-// then records the transition number in *stall and this and every later
-// fused launch does nothing: the host finishes the last completed
-// transition and runs the stalled ones again through the four launches.
+// in) and marks its receivers outside the set with flag bit 1.  sc_width_f
+// then runs the width, temperature and removal processes over the new
+// list, applies the steering's border reset to the cells it visits (the
+// others hold h = T = 0 already), writes flags 0/1, and marks the vents
+// whose cell stayed in the set for the next transition's emission.
+//
+// Border cells of the set.  A cell of the border mask is never removed, and
+// after every steering it holds h = T = 0 and zero flows; unless a vent
+// emits into it, every later transition leaves it so: it sends nothing
+// (h = 0), what it receives the steering removes, and the steering's reset
+// rewrites the zeros it holds.  So sc_flows_f writes its zero flows once
+// and then leaves it out of the next list (the reference's example keeps
+// the whole border in its set, which would otherwise be revisited at every
+// step); *skip counts those cells, which stay in the set (flag 1).  The
+// engine turns this off when a vent lies on the border.
+//
+// When the set is empty OpenCAL's processes cover the whole grid.  The set
+// at the start of transition t is empty exactly when no cell of P is a
+// member and no border cell was set aside: then the new list is empty and
+// *skip is 0, and sc_width_f records t in *stall.  sc_flows_f of t has then
+// only finished transition t - 1 (T copied back), and every later fused
+// launch does nothing: the host finishes the last completed transition and
+// runs the stalled ones again through the four launches.
 struct FusedCtl {
 	const int* P;          // the list of the previous transition's width
 	int* Lnew;             // the list of this transition
 	int* cnt;              // [3] list lengths by transition mod 3
-	int* surv;             // [3] survivors by transition mod 3
+	int* skip;             // border cells of the set left out of the lists
 	int* stall;            // transition at which the set was found empty (0: none)
 	int* vent_act;         // [VENTS_PER_LAUNCH] transition of each vent's last emission
 	const int* count0;     // first fused transition: P is the set list, count0[0] entries
 	int t;                 // transition number (>= 1)
 	int entry;             // first fused transition: T is current, P = the set
+	int skip_border;       // leave idle border cells of the set out of the lists
 };
 
 __device__ __forceinline__ int vent_index(const VentArgs& v, int c)
@@ -421,33 +437,30 @@ __device__ __forceinline__ int vent_index(const VentArgs& v, int c)
 	return r;
 }
 
+// Eight lanes per cell (lane k - 1 of a group owns direction k); a block
+// takes SC_BLOCK / 8 consecutive entries of P per pass, so that the
+// positions of its cells in the next list come from one atomic per pass.
 __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict__ z, const double* __restrict__ h,
                                                        double* __restrict__ T, const double* __restrict__ Tn, Flows8 f,
-                                                       uint8_t* flags, FusedCtl F, VentArgs v, int rows, int cols,
-                                                       SciaraParams P)
+                                                       uint8_t* flags, const uint8_t* __restrict__ Mb, FusedCtl F,
+                                                       VentArgs v, int rows, int cols, SciaraParams P)
 {
-	const int tp = (F.t + 2) % 3, tc = F.t % 3, tn = (F.t + 1) % 3;
+	constexpr int CPB = SC_BLOCK / 8;            // cells per block and pass
+	constexpr int NW = SC_BLOCK / dev::WAVE;     // waves per block
+	__shared__ int s_off[2][NW];
+	__shared__ int s_base[2];
+	const int tc = F.t % 3, tn = (F.t + 1) % 3, tp = (F.t + 2) % 3;
 	if (*F.stall) return;
 	const int np = F.entry ? F.count0[0] : F.cnt[tp];
-	const int sp = F.entry ? F.count0[0] : F.surv[tp];
-	if (sp == 0) {
-		if (blockIdx.x == 0 && threadIdx.x == 0) *F.stall = F.t;
-		return;
-	}
-	if (blockIdx.x == 0 && threadIdx.x == 0) {
-		F.cnt[tn] = 0;    // the next transition's list length
-		F.surv[tc] = 0;   // this transition's survivors (sc_width_f)
-	}
-	const int lane = (int)(threadIdx.x & 63);
+	if (blockIdx.x == 0 && threadIdx.x == 0) F.cnt[tn] = 0;   // the next transition's list length
+	const int lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
 	const int g0 = lane & ~7;
 	const int k = (lane & 7) + 1;
 	const bool writer = (lane & 7) == 0;
-	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
-	const size_t idx0 = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3);
-	// a wave's groups agree on their trip count: the loop runs to the end of
-	// the wave's last group so that the survivors' ballot is wave-uniform
-	for (size_t base = idx0 & ~(size_t)7; base < (size_t)np; base += stride) {
-		const size_t idx = base + ((threadIdx.x >> 3) & 7);
+	const unsigned long long below = (1ull << lane) - 1ull;
+	int pass = 0;
+	for (size_t bb = (size_t)blockIdx.x * CPB; bb < (size_t)np; bb += (size_t)gridDim.x * CPB, pass ^= 1) {
+		const size_t idx = bb + (threadIdx.x >> 3);
 		const bool valid = idx < (size_t)np;
 		const int c = valid ? F.P[idx] : 0;
 		const unsigned fl = valid ? flags[c] : 0u;
@@ -457,24 +470,34 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 			if (!F.entry && writer) T[c] = t;
 		}
 		const bool member = valid && (fl & 1u);
-		// survivors to the new list: one atomic per wave
-		const unsigned long long wm = __ballot(member && writer);
-		int slot = 0;
-		if (wm) {
-			int basepos = 0;
-			if (lane == 0) basepos = atomicAdd(F.cnt + tc, __popcll(wm));
-			basepos = __shfl(basepos, 0, 64);
-			slot = basepos + __popcll(wm & ((1ull << lane) - 1ull));
-		}
-		if (member && writer) F.Lnew[slot] = c;
-		if (!member) continue;
-		const int vk = vent_index(v, c);
-		double h0 = h[c];
+		const int vk = member ? vent_index(v, c) : -1;
+		double h0 = member ? h[c] : 0.0;
 		if (vk >= 0) {
 			h0 = h0 + v.e[vk];
 			t = P.PTvent;
 			if (writer) F.vent_act[vk] = F.t;
 		}
+		const bool set_aside = member && F.skip_border && vk < 0 && !(h0 > 0) && Mb[c] != 0;
+		const bool keep = member && !set_aside;
+		// positions in the next list: one atomic per block and pass
+		const unsigned long long km = __ballot(keep && writer);
+		const unsigned long long sm = __ballot(set_aside && writer);
+		if (lane == 0) s_off[pass][wid] = __popcll(km) | (__popcll(sm) << 16);
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			int tot = 0, sk = 0;
+			for (int w = 0; w < NW; ++w) {
+				const int x = s_off[pass][w];
+				s_off[pass][w] = tot;
+				tot += x & 0xffff;
+				sk += x >> 16;
+			}
+			s_base[pass] = tot ? atomicAdd(F.cnt + tc, tot) : 0;
+			if (sk) atomicAdd(F.skip, sk);
+		}
+		__syncthreads();
+		if (keep && writer) F.Lnew[s_base[pass] + s_off[pass][wid] + __popcll(km & below)] = c;
+		if (!member) continue;
 		if (!(h0 > 0)) {
 			f.f[k - 1][c] = 0.0;
 			continue;
@@ -501,7 +524,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 		H[0] = sz0;
 #pragma unroll
 		for (int q = 1; q < 9; ++q) H[q] = __shfl(Hk, g0 + q - 1, 64);
-		unsigned keep = 1u | ((unsigned)((__ballot((int)down) >> g0) & 0xffull) << 1);
+		unsigned keepm = 1u | ((unsigned)((__ballot((int)down) >> g0) & 0xffull) << 1);
 		double avg;
 		bool loop;
 		do {
@@ -510,24 +533,26 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 			int counter = 0;
 #pragma unroll
 			for (int q = 0; q < 9; ++q)
-				if ((keep >> q) & 1u) {
+				if ((keepm >> q) & 1u) {
 					avg += H[q];
 					counter++;
 				}
 			if (counter != 0) avg = avg / (double)counter;
 #pragma unroll
 			for (int q = 0; q < 9; ++q)
-				if (((keep >> q) & 1u) && avg <= H[q]) {
-					keep &= ~(1u << q);
+				if (((keepm >> q) & 1u) && avg <= H[q]) {
+					keepm &= ~(1u << q);
 					loop = true;
 				}
 		} while (loop);
 		double fk = 0.0;
-		if (((keep >> k) & 1u) && h0 > hcck) fk = Pr * (avg - Hk);
+		if (((keepm >> k) & 1u) && h0 > hcck) fk = Pr * (avg - Hk);
 		// the steering reset every flow of the set's cells; the process
 		// writes the positive ones (Sciara.cpp:95-101)
 		f.f[k - 1][c] = fk > 0 ? fk : 0.0;
-		if (fk > 0 && nbk >= 0) {
+		// a receiver outside the set joins the next list once (flag bit 1);
+		// the set's cells are listed above
+		if (fk > 0 && nbk >= 0 && !(flags[nbk] & 1u)) {
 			unsigned* w = reinterpret_cast<unsigned*>(flags + (nbk & ~3));
 			const int sh = (nbk & 3) * 8;
 			const unsigned old = atomicOr(w, 2u << sh);
@@ -546,6 +571,11 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width_f(double* __restrict__ z, d
 	if (*F.stall) return;
 	const int tc = F.t % 3;
 	const int cnt = F.cnt[tc];
+	if (cnt == 0) {
+		// no member in P: the set is empty unless border cells were set aside
+		if (*F.skip == 0 && blockIdx.x == 0 && threadIdx.x == 0) *F.stall = F.t;
+		return;
+	}
 	const int lane = (int)(threadIdx.x & 63);
 	const int g0 = lane & ~7;
 	const int k = (lane & 7) + 1;
@@ -559,73 +589,66 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width_f(double* __restrict__ z, d
 			fout = f.f[q];
 		}
 	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
-	const size_t idx0 = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3);
-	for (size_t base = idx0 & ~(size_t)7; base < (size_t)cnt; base += stride) {
-		const size_t idx = base + ((threadIdx.x >> 3) & 7);
-		const bool valid = idx < (size_t)cnt;
-		bool surv = false;
-		if (valid) {
-			const int c = F.Lnew[idx];
-			const int i = c / cols, j = c - (c / cols) * cols;
-			double ih = h[c], it = T[c];
-			const int vk = vent_index(v, c);
-			if (vk >= 0 && F.vent_act[vk] == F.t) {
-				ih = ih + v.e[vk];
-				it = P.PTvent;
-			}
-			const int qk = nb_flat(i, j, k, rows, cols);
-			const double in_k = qk >= 0 ? fin[qk] : 0.0;
-			const double out_k = fout[c];
-			double nt_k = 0.0;
-			if (qk >= 0) {
-				nt_k = T[qk];
-				const int vn = vent_index(v, qk);
-				if (vn >= 0 && F.vent_act[vn] == F.t) nt_k = P.PTvent;
-			}
-			const double ht_k = in_k * nt_k;
-			double ht = 0.0, in_sum = 0.0, out_sum = 0.0;
+	for (size_t idx = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3); idx < (size_t)cnt;
+	     idx += stride) {
+		const int c = F.Lnew[idx];
+		const int i = c / cols, j = c - (c / cols) * cols;
+		double ih = h[c], it = T[c];
+		const int vk = vent_index(v, c);
+		if (vk >= 0 && F.vent_act[vk] == F.t) {
+			ih = ih + v.e[vk];
+			it = P.PTvent;
+		}
+		const int qk = nb_flat(i, j, k, rows, cols);
+		const double in_k = qk >= 0 ? fin[qk] : 0.0;
+		const double out_k = fout[c];
+		double nt_k = 0.0;
+		if (qk >= 0) {
+			nt_k = T[qk];
+			const int vn = vent_index(v, qk);
+			if (vn >= 0 && F.vent_act[vn] == F.t) nt_k = P.PTvent;
+		}
+		const double ht_k = in_k * nt_k;
+		double ht = 0.0, in_sum = 0.0, out_sum = 0.0;
 #pragma unroll
-			for (int m = 0; m < 8; ++m) {
-				ht += __shfl(ht_k, g0 + m, 64);
-				in_sum += __shfl(in_k, g0 + m, 64);
-				out_sum += __shfl(out_k, g0 + m, 64);
-			}
-			double h1 = ih;
-			h1 += in_sum - out_sum;
-			double t1 = it;
-			if (in_sum > 0 || out_sum > 0) {
-				double residual = ih;
-				residual -= out_sum;
-				t1 = (residual * it + ht) / (residual + in_sum);
-			}
-			double h2 = h1, t2 = t1;
-			const bool mb = Mb[c] != 0;
-			if (h1 > 0 && !mb) {
-				const double nT = t1;
-				const double num = (((3.0 * cr_cube(nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
-				const double den = ((P.Prho * P.Pcv) * h1) * P.Pac;
-				const double aus = 1.0 + num / den;
-				const double st = nT / cr_pow_third(aus);
-				t2 = st;
-				if (st <= P.PTsol) {
-					if (writer) {
-						z[c] = z[c] + h1;
-						Msl[c] = Msl[c] + h1;
-					}
-					h2 = 0.0;
-					t2 = P.PTsol;
+		for (int m = 0; m < 8; ++m) {
+			ht += __shfl(ht_k, g0 + m, 64);
+			in_sum += __shfl(in_k, g0 + m, 64);
+			out_sum += __shfl(out_k, g0 + m, 64);
+		}
+		double h1 = ih;
+		h1 += in_sum - out_sum;
+		double t1 = it;
+		if (in_sum > 0 || out_sum > 0) {
+			double residual = ih;
+			residual -= out_sum;
+			t1 = (residual * it + ht) / (residual + in_sum);
+		}
+		double h2 = h1, t2 = t1;
+		const bool mb = Mb[c] != 0;
+		if (h1 > 0 && !mb) {
+			const double nT = t1;
+			const double num = (((3.0 * cr_cube(nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
+			const double den = ((P.Prho * P.Pcv) * h1) * P.Pac;
+			const double aus = 1.0 + num / den;
+			const double st = nT / cr_pow_third(aus);
+			t2 = st;
+			if (st <= P.PTsol) {
+				if (writer) {
+					z[c] = z[c] + h1;
+					Msl[c] = Msl[c] + h1;
 				}
-			}
-			surv = !(t2 <= P.PTsol && !mb);
-			if (writer) {
-				// the steering's border reset (Sciara.cpp:189-194) after the removal
-				h[c] = mb ? 0.0 : h2;
-				Tn[c] = mb ? 0.0 : t2;
-				flags[c] = surv ? 1 : 0;
+				h2 = 0.0;
+				t2 = P.PTsol;
 			}
 		}
-		const unsigned long long wm = __ballot(surv && writer);
-		if (lane == 0 && wm) atomicAdd(F.surv + tc, __popcll(wm));
+		const bool surv = !(t2 <= P.PTsol && !mb);
+		if (writer) {
+			// the steering's border reset (Sciara.cpp:189-194) after the removal
+			h[c] = mb ? 0.0 : h2;
+			Tn[c] = mb ? 0.0 : t2;
+			flags[c] = surv ? 1 : 0;
+		}
 	}
 }
 
@@ -697,7 +720,8 @@ struct SciaraProgram : Program {
 	// fused transitions (sc_flows_f / sc_width_f)
 	bool fz_on = false;
 	int fz_t = 1;                  // number of the next fused transition (from 1)
-	int* fz_buf = nullptr;         // cnt[3], surv[3], stall, vent_act[VENTS_PER_LAUNCH]
+	int* fz_buf = nullptr;         // cnt[3], skip, stall, vent_act[VENTS_PER_LAUNCH]
+	bool fz_skip_border = false;   // no vent on the border mask: idle border cells are set aside
 	int* fz_lists[2] = {nullptr, nullptr};   // the list buffers (d_cells, alt at entry)
 	std::vector<double> fz_elapsed;           // elapsed time at the start of each fused transition
 	bool fz_replaying = false;                // running stalled transitions again (four launches)
@@ -867,10 +891,9 @@ struct SciaraProgram : Program {
 
 	bool fz_eligible(ModelX* m) const
 	{
-		// opt-in: a 2048 x 2048 run of this path ended in a host-side crash
-		// whose cause is not found yet; the four launches stay the default
-		const char* on = getenv("CALHIP_SCIARA_TWO_LAUNCH");
-		if (!active || fz_replaying || m->flags_dirty || !on || strcmp(on, "1") != 0) return false;
+		// CALHIP_SCIARA_TWO_LAUNCH=0 keeps every transition in four launches
+		const char* sw = getenv("CALHIP_SCIARA_TWO_LAUNCH");
+		if (!active || fz_replaying || m->flags_dirty || (sw && strcmp(sw, "0") == 0)) return false;
 		std::vector<std::pair<int, double>> vl;
 		vent_list(vl, m->pub.columns);
 		return vl.size() <= (size_t)VENTS_PER_LAUNCH;
@@ -883,12 +906,13 @@ struct SciaraProgram : Program {
 		F.P = fz_lists[s % 2];
 		F.Lnew = fz_lists[(s + 1) % 2];
 		F.cnt = fz_buf;
-		F.surv = fz_buf + 3;
-		F.stall = fz_buf + 6;
-		F.vent_act = fz_buf + 7;
+		F.skip = fz_buf + 3;
+		F.stall = fz_buf + 4;
+		F.vent_act = fz_buf + 5;
 		F.count0 = m->d_count;
 		F.t = t;
 		F.entry = t == 1 ? 1 : 0;
+		F.skip_border = fz_skip_border ? 1 : 0;
 		return F;
 	}
 
@@ -897,11 +921,17 @@ struct SciaraProgram : Program {
 		if (check_entry(m)) return -1;
 		refresh_estimate();
 		if (!fz_on) {
-			const size_t bytes = sizeof(int) * (7 + VENTS_PER_LAUNCH);
+			const size_t bytes = sizeof(int) * (5 + VENTS_PER_LAUNCH);
 			if (!fz_buf && CALHIP_CHECK(hipMalloc(&fz_buf, bytes))) return -1;
 			if (CALHIP_CHECK(hipMemsetAsync(fz_buf, 0, bytes, m->stream))) return -1;
 			fz_lists[0] = m->d_cells;
 			fz_lists[1] = alt;
+			// the border mask is read-only for the program: its host copy is
+			// the one uploaded
+			const CALbyte* hm = mb->pub.b.current;
+			fz_skip_border = true;
+			for (int v = 0; v < b.n_vents; ++v)
+				if (hm[(size_t)vent_rc[2 * v] * m->pub.columns + vent_rc[2 * v + 1]]) fz_skip_border = false;
 			fz_t = 1;
 			fz_elapsed.clear();
 			fz_on = true;
@@ -923,7 +953,7 @@ struct SciaraProgram : Program {
 		timer_begin(m, &ev);
 		hipLaunchKernelGGL(sc_flows_f, dim3(grid8), dim3(SC_BLOCK), 0, m->stream, (const double*)z->d_cur,
 		                   (const double*)h->d_cur, (double*)T->d_cur, (const double*)T->d_nxt, flows(), m->d_flags,
-		                   F, va, rows, cols, P);
+		                   (const uint8_t*)mb->d_cur, F, va, rows, cols, P);
 		hipLaunchKernelGGL(sc_width_f, dim3(grid8), dim3(SC_BLOCK), 0, m->stream, (double*)z->d_cur,
 		                   (double*)h->d_cur, (const double*)T->d_cur, (double*)T->d_nxt, flows(),
 		                   (const uint8_t*)mb->d_cur, (double*)msl->d_cur, m->d_flags, F, va, rows, cols, P);
@@ -946,16 +976,19 @@ struct SciaraProgram : Program {
 
 	// Back to the four-launch representation: the last completed fused
 	// transition finished by sc_finish (temperatures copied back, the
-	// steering's flow reset, the set compacted into the list); transitions
+	// steering's flow reset, the set compacted into the list), the list
+	// rebuilt from the flags when border cells were set aside; transitions
 	// that found the set empty run again through the four launches.
 	int fz_finish(ModelX* m)
 	{
 		if (!fz_on) return 0;
 		fz_on = false;
-		int stall = 0;
-		if (CALHIP_CHECK(hipMemcpyAsync(&stall, fz_buf + 6, sizeof(int), hipMemcpyDeviceToHost, m->stream)) ||
+		int skip_stall[2] = {0, 0};
+		if (CALHIP_CHECK(hipMemcpyAsync(skip_stall, fz_buf + 3, sizeof(skip_stall), hipMemcpyDeviceToHost,
+		                                m->stream)) ||
 		    CALHIP_CHECK(hipStreamSynchronize(m->stream)))
 			return -1;
+		const int skipped = skip_stall[0], stall = skip_stall[1];
 		const int last = stall ? stall - 1 : fz_t - 1;   // last completed fused transition
 		if (last >= 1) {
 			int* done_list = fz_lists[last % 2];
@@ -970,6 +1003,11 @@ struct SciaraProgram : Program {
 			m->d_cells = other;
 			alt = done_list;
 			unordered = true;
+			if (skipped) {
+				// the set-aside border cells back into the row-major list
+				if (k::rebuild_active(m)) return -1;
+				unordered = false;
+			}
 		} else {
 			m->d_cells = fz_lists[0];
 			alt = fz_lists[1];

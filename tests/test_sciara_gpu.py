@@ -126,8 +126,8 @@ def test_sciara_properties_1024():
     assert (msl > 0).sum() > 0
 
 
-# the two-launch transitions are opt-in (CALHIP_SCIARA_TWO_LAUNCH=1); the
-# four launches are the default
+# active runs with the steering twin take two launches per transition;
+# CALHIP_SCIARA_TWO_LAUNCH=0 keeps them in four
 TWO = {"CALHIP_SCIARA_TWO_LAUNCH": "1"}
 FOUR = {"CALHIP_SCIARA_TWO_LAUNCH": "0"}
 
@@ -231,3 +231,46 @@ def test_sciara_two_launch_set_empties():
         assert np.array_equal(a[k], b[k]), f"vent alone {k}"
     assert a["active"] == b["active"] == 0 and a["elapsed"] == b["elapsed"]
     assert 0 < a["reruns"] < 90
+
+
+def test_sciara_two_launch_border_only_set():
+    """The set keeps the border cells (the example's init adds the border
+    ring) and the lava solidifies fast, so the set ends as the border alone:
+    the two-launch transitions set the idle border cells aside after their
+    first transition and must neither stall nor re-run (the set is not
+    empty); the state and the set equal the four-launch path's."""
+    inp = _inputs(64, 96, 7, [[14, 40]], steps_per_rate=20, n_rates=5)
+    rates = inp["rates"].copy()
+    rates[:, 1:] = 0.0
+    inp["rates"] = np.ascontiguousarray(rates)
+    prm = inp["params"].copy()
+    prm[1] = 1359.0
+    inp["params"] = prm
+    for steps, chunk in ((100, None), (100, 7)):
+        print(f"case sciara border-only set, {steps} steps, reads every {chunk or steps}", flush=True)
+        a, la = _run_steps(inp, steps, TWO, chunk=chunk)
+        b, lb = _run_steps(inp, steps, FOUR, chunk=chunk)
+        print(f"launches two-launch {la}, four-launch {lb}, active {a['active']}, reruns {a['reruns']}", flush=True)
+        for k in ("z", "h", "T", "Msl"):
+            assert np.array_equal(a[k], b[k]), f"chunk={chunk} {k}"
+        assert a["active"] == b["active"] == 2 * (64 + 96) - 4   # the border ring
+        assert a["elapsed"] == b["elapsed"] and a["reruns"] == 0
+    ora = native.oracle_sciara(inp, 100, True, crmath=True)
+    for k in ("z", "h", "T", "Msl"):
+        assert np.array_equal(a[k], ora[k]), k
+
+
+def test_sciara_two_launch_vent_on_border():
+    """A vent on the border mask: its cell emits (h + e > 0) while it is a
+    border cell, so the border is not set aside; bit-exact against the
+    four-launch path and the crmath oracle."""
+    inp = _inputs(64, 96, 5, [[0, 40], [20, 60]], steps_per_rate=40, n_rates=3)
+    steps = 120
+    a, la = _run_steps(inp, steps, TWO)
+    b, lb = _run_steps(inp, steps, FOUR)
+    ora = native.oracle_sciara(inp, steps, True, crmath=True)
+    print(f"launches two-launch {la}, four-launch {lb}, active {a['active']}", flush=True)
+    for k in ("z", "h", "T", "Msl"):
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], ora[k]), k
+    assert a["active"] == b["active"] == ora["active"]
