@@ -492,6 +492,22 @@ def ref_leg_child(args, timeout_s=240):
     return None, f"exit status {p.returncode}"
 
 
+def live_children():
+    """Processes this run started that are still alive (each child of this
+    script is waited for; a non-empty list names what a library left)."""
+    try:
+        import psutil
+    except ImportError:
+        return None
+    out = []
+    for p in psutil.Process().children(recursive=True):
+        try:
+            out.append({"pid": p.pid, "name": p.name(), "cmdline": " ".join(p.cmdline())[:200]})
+        except psutil.Error:
+            pass
+    return out
+
+
 def run_workload_child(name, timeout_s=300):
     """One secondary workload in a child process (`bench.py --only name`),
     waited for before returning: a crash or a time-out there becomes an error
@@ -622,6 +638,7 @@ def main():
             wl[name] = run_workload_child(name)
             log(f"{name}: {json.dumps(wl[name])} ({time.perf_counter() - t0:.1f} s)")
         out["workloads"] = wl
+    out["children_at_exit"] = live_children()
     print(json.dumps(out), flush=True)
 
 

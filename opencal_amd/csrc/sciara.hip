@@ -440,6 +440,10 @@ __device__ __forceinline__ int vent_index(const VentArgs& v, int c)
 // Eight lanes per cell (lane k - 1 of a group owns direction k); a block
 // takes SC_BLOCK / 8 consecutive entries of P per pass, so that the
 // positions of its cells in the next list come from one atomic per pass.
+// A pass is latency-bound (a few cells per wave): its loads are issued
+// together, the list atomic's return is awaited only after the flows, and
+// the two power laws of a cell are split between the group's even and odd
+// lanes.
 __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict__ z, const double* __restrict__ h,
                                                        double* __restrict__ T, const double* __restrict__ Tn, Flows8 f,
                                                        uint8_t* flags, const uint8_t* __restrict__ Mb, FusedCtl F,
@@ -463,27 +467,43 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 		const size_t idx = bb + (threadIdx.x >> 3);
 		const bool valid = idx < (size_t)np;
 		const int c = valid ? F.P[idx] : 0;
-		const unsigned fl = valid ? flags[c] : 0u;
-		double t = 0.0;
+		const int i = c / cols, j = c - (c / cols) * cols;
+		const int nbk = valid ? nb_flat(i, j, k, rows, cols) : -1;
+		// the cell and its neighbour k (h, z and flags are not written by
+		// this launch except flag bit 1, which is not read here)
+		unsigned fl = 0u, fln = 0u;
+		double tcur = 0.0, hcur = 0.0, sz0 = 0.0, hk = 0.0, zr = 0.0;
+		bool mbc = false;
 		if (valid) {
-			t = F.entry ? T[c] : Tn[c];
-			if (!F.entry && writer) T[c] = t;
+			fl = flags[c];
+			tcur = F.entry ? T[c] : Tn[c];
+			hcur = h[c];
+			sz0 = z[c];
+			mbc = Mb[c] != 0;
+			if (nbk >= 0) {
+				hk = h[nbk];
+				zr = z[nbk];
+				fln = flags[nbk];
+			}
 		}
+		if (valid && !F.entry && writer) T[c] = tcur;
 		const bool member = valid && (fl & 1u);
 		const int vk = member ? vent_index(v, c) : -1;
-		double h0 = member ? h[c] : 0.0;
+		double h0 = member ? hcur : 0.0, t = tcur;
 		if (vk >= 0) {
 			h0 = h0 + v.e[vk];
 			t = P.PTvent;
 			if (writer) F.vent_act[vk] = F.t;
 		}
-		const bool set_aside = member && F.skip_border && vk < 0 && !(h0 > 0) && Mb[c] != 0;
+		const bool wet = member && h0 > 0;
+		const bool set_aside = member && F.skip_border && vk < 0 && !wet && mbc;
 		const bool keep = member && !set_aside;
 		// positions in the next list: one atomic per block and pass
 		const unsigned long long km = __ballot(keep && writer);
 		const unsigned long long sm = __ballot(set_aside && writer);
 		if (lane == 0) s_off[pass][wid] = __popcll(km) | (__popcll(sm) << 16);
 		__syncthreads();
+		int base = 0;
 		if (threadIdx.x == 0) {
 			int tot = 0, sk = 0;
 			for (int w = 0; w < NW; ++w) {
@@ -492,72 +512,66 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 				tot += x & 0xffff;
 				sk += x >> 16;
 			}
-			s_base[pass] = tot ? atomicAdd(F.cnt + tc, tot) : 0;
+			if (tot) base = atomicAdd(F.cnt + tc, tot);
 			if (sk) atomicAdd(F.skip, sk);
 		}
+		if (member && !wet) f.f[k - 1][c] = 0.0;
+		if (wet) {
+			const double pw = cr_pow10((lane & 1) ? P.c + P.d * t : P.a + P.b * t);
+			const double Pr = __shfl(pw, g0, 64);
+			const double hc = __shfl(pw, g0 + 1, 64);
+			if (nbk >= 0) {
+				const int vn = vent_index(v, nbk);
+				if (vn >= 0 && (fln & 1u)) hk = hk + v.e[vn];
+			}
+			const double zk = k < 5 ? zr : sz0 - (sz0 - zr) / P.rad2;
+			const double lvl0 = sz0 + h0;
+			const double lk = zk + hk;
+			const bool down = lvl0 > lk;
+			const double Hk = down ? lk : 0.0;
+			const double hcck = hc * cr_cos(cr_atan((lvl0 - lk) / P.Pc));
+			double H[9];
+			H[0] = sz0;
+#pragma unroll
+			for (int q = 1; q < 9; ++q) H[q] = __shfl(Hk, g0 + q - 1, 64);
+			unsigned keepm = 1u | ((unsigned)((__ballot((int)down) >> g0) & 0xffull) << 1);
+			double avg;
+			bool loop;
+			do {
+				loop = false;
+				avg = h0;
+				int counter = 0;
+#pragma unroll
+				for (int q = 0; q < 9; ++q)
+					if ((keepm >> q) & 1u) {
+						avg += H[q];
+						counter++;
+					}
+				if (counter != 0) avg = avg / (double)counter;
+#pragma unroll
+				for (int q = 0; q < 9; ++q)
+					if (((keepm >> q) & 1u) && avg <= H[q]) {
+						keepm &= ~(1u << q);
+						loop = true;
+					}
+			} while (loop);
+			double fk = 0.0;
+			if (((keepm >> k) & 1u) && h0 > hcck) fk = Pr * (avg - Hk);
+			// the steering reset every flow of the set's cells; the process
+			// writes the positive ones (Sciara.cpp:95-101)
+			f.f[k - 1][c] = fk > 0 ? fk : 0.0;
+			// a receiver outside the set joins the next list once (flag bit
+			// 1); the set's cells are listed below
+			if (fk > 0 && nbk >= 0 && !(fln & 1u)) {
+				unsigned* w = reinterpret_cast<unsigned*>(flags + (nbk & ~3));
+				const int sh = (nbk & 3) * 8;
+				const unsigned old = atomicOr(w, 2u << sh);
+				if (((old >> sh) & 3u) == 0u) F.Lnew[atomicAdd(F.cnt + tc, 1)] = nbk;
+			}
+		}
+		if (threadIdx.x == 0) s_base[pass] = base;
 		__syncthreads();
 		if (keep && writer) F.Lnew[s_base[pass] + s_off[pass][wid] + __popcll(km & below)] = c;
-		if (!member) continue;
-		if (!(h0 > 0)) {
-			f.f[k - 1][c] = 0.0;
-			continue;
-		}
-		const int i = c / cols, j = c - (c / cols) * cols;
-		const double Pr = cr_pow10(P.a + P.b * t);
-		const double hc = cr_pow10(P.c + P.d * t);
-		const double sz0 = z[c];
-		const int nbk = nb_flat(i, j, k, rows, cols);
-		double hk = 0.0, zr = 0.0;
-		if (nbk >= 0) {
-			hk = h[nbk];
-			zr = z[nbk];
-			const int vn = vent_index(v, nbk);
-			if (vn >= 0 && (flags[nbk] & 1u)) hk = hk + v.e[vn];
-		}
-		const double zk = k < 5 ? zr : sz0 - (sz0 - zr) / P.rad2;
-		const double lvl0 = sz0 + h0;
-		const double lk = zk + hk;
-		const bool down = lvl0 > lk;
-		const double Hk = down ? lk : 0.0;
-		const double hcck = hc * cr_cos(cr_atan((lvl0 - lk) / P.Pc));
-		double H[9];
-		H[0] = sz0;
-#pragma unroll
-		for (int q = 1; q < 9; ++q) H[q] = __shfl(Hk, g0 + q - 1, 64);
-		unsigned keepm = 1u | ((unsigned)((__ballot((int)down) >> g0) & 0xffull) << 1);
-		double avg;
-		bool loop;
-		do {
-			loop = false;
-			avg = h0;
-			int counter = 0;
-#pragma unroll
-			for (int q = 0; q < 9; ++q)
-				if ((keepm >> q) & 1u) {
-					avg += H[q];
-					counter++;
-				}
-			if (counter != 0) avg = avg / (double)counter;
-#pragma unroll
-			for (int q = 0; q < 9; ++q)
-				if (((keepm >> q) & 1u) && avg <= H[q]) {
-					keepm &= ~(1u << q);
-					loop = true;
-				}
-		} while (loop);
-		double fk = 0.0;
-		if (((keepm >> k) & 1u) && h0 > hcck) fk = Pr * (avg - Hk);
-		// the steering reset every flow of the set's cells; the process
-		// writes the positive ones (Sciara.cpp:95-101)
-		f.f[k - 1][c] = fk > 0 ? fk : 0.0;
-		// a receiver outside the set joins the next list once (flag bit 1);
-		// the set's cells are listed above
-		if (fk > 0 && nbk >= 0 && !(flags[nbk] & 1u)) {
-			unsigned* w = reinterpret_cast<unsigned*>(flags + (nbk & ~3));
-			const int sh = (nbk & 3) * 8;
-			const unsigned old = atomicOr(w, 2u << sh);
-			if (((old >> sh) & 3u) == 0u) F.Lnew[atomicAdd(F.cnt + tc, 1)] = nbk;
-		}
 	}
 }
 
