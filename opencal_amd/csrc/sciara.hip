@@ -437,13 +437,51 @@ __device__ __forceinline__ int vent_index(const VentArgs& v, int c)
 	return r;
 }
 
+// What a lane of sc_flows_f reads of its cell c and of c's neighbour k (h,
+// z and flags are not written by that launch except flag bit 1, which is
+// not read there).
+struct FlowsIn {
+	int c, nbk;
+	unsigned fl, fln;
+	double tcur, hcur, sz0, hk, zr;
+	bool mbc;
+};
+
+__device__ __forceinline__ FlowsIn flows_load(int c, int k, const double* __restrict__ z,
+                                              const double* __restrict__ h, const double* T,
+                                              const double* __restrict__ Tn, const uint8_t* flags,
+                                              const uint8_t* __restrict__ Mb, int entry, int rows, int cols)
+{
+	FlowsIn r;
+	r.c = c;
+	const int i = c / cols, j = c - (c / cols) * cols;
+	r.nbk = nb_flat(i, j, k, rows, cols);
+	r.fl = flags[c];
+	r.tcur = entry ? T[c] : Tn[c];
+	r.hcur = h[c];
+	r.sz0 = z[c];
+	r.mbc = Mb[c] != 0;
+	r.hk = 0.0;
+	r.zr = 0.0;
+	r.fln = 0u;
+	if (r.nbk >= 0) {
+		r.hk = h[r.nbk];
+		r.zr = z[r.nbk];
+		r.fln = flags[r.nbk];
+	}
+	return r;
+}
+
 // Eight lanes per cell (lane k - 1 of a group owns direction k); a block
 // takes SC_BLOCK / 8 consecutive entries of P per pass, so that the
 // positions of its cells in the next list come from one atomic per pass.
-// A pass is latency-bound (a few cells per wave): its loads are issued
-// together, the list atomic's return is awaited only after the flows, and
-// the two power laws of a cell are split between the group's even and odd
-// lanes.
+// A pass is latency-bound (a few cells per wave): the first pass's list
+// entry and the cell's loads go out before the list length and the stall
+// word are known (an entry beyond the list, stale or never written, is
+// clamped into the grid and its values are not used), a pass's loads are
+// issued together, the list atomic's return is awaited only after the
+// flows, and the two power laws of a cell are split between the group's
+// even and odd lanes.
 __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict__ z, const double* __restrict__ h,
                                                        double* __restrict__ T, const double* __restrict__ Tn, Flows8 f,
                                                        uint8_t* flags, const uint8_t* __restrict__ Mb, FusedCtl F,
@@ -454,38 +492,30 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 	__shared__ int s_off[2][NW];
 	__shared__ int s_base[2];
 	const int tc = F.t % 3, tn = (F.t + 1) % 3, tp = (F.t + 2) % 3;
-	if (*F.stall) return;
-	const int np = F.entry ? F.count0[0] : F.cnt[tp];
-	if (blockIdx.x == 0 && threadIdx.x == 0) F.cnt[tn] = 0;   // the next transition's list length
 	const int lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
 	const int g0 = lane & ~7;
 	const int k = (lane & 7) + 1;
 	const bool writer = (lane & 7) == 0;
 	const unsigned long long below = (1ull << lane) - 1ull;
+	const int ncell = rows * cols;
+	const size_t first = (size_t)blockIdx.x * CPB;
+	int c_pre = F.P[min(first + (threadIdx.x >> 3), (size_t)ncell - 1)];
+	c_pre = min(max(c_pre, 0), ncell - 1);
+	const FlowsIn pre = flows_load(c_pre, k, z, h, T, Tn, flags, Mb, F.entry, rows, cols);
+	if (*F.stall) return;
+	const int np = F.entry ? F.count0[0] : F.cnt[tp];
+	if (blockIdx.x == 0 && threadIdx.x == 0) F.cnt[tn] = 0;   // the next transition's list length
 	int pass = 0;
-	for (size_t bb = (size_t)blockIdx.x * CPB; bb < (size_t)np; bb += (size_t)gridDim.x * CPB, pass ^= 1) {
+	for (size_t bb = first; bb < (size_t)np; bb += (size_t)gridDim.x * CPB, pass ^= 1) {
 		const size_t idx = bb + (threadIdx.x >> 3);
 		const bool valid = idx < (size_t)np;
-		const int c = valid ? F.P[idx] : 0;
-		const int i = c / cols, j = c - (c / cols) * cols;
-		const int nbk = valid ? nb_flat(i, j, k, rows, cols) : -1;
-		// the cell and its neighbour k (h, z and flags are not written by
-		// this launch except flag bit 1, which is not read here)
-		unsigned fl = 0u, fln = 0u;
-		double tcur = 0.0, hcur = 0.0, sz0 = 0.0, hk = 0.0, zr = 0.0;
-		bool mbc = false;
-		if (valid) {
-			fl = flags[c];
-			tcur = F.entry ? T[c] : Tn[c];
-			hcur = h[c];
-			sz0 = z[c];
-			mbc = Mb[c] != 0;
-			if (nbk >= 0) {
-				hk = h[nbk];
-				zr = z[nbk];
-				fln = flags[nbk];
-			}
-		}
+		const FlowsIn in = bb == first ? pre
+		                               : flows_load(valid ? F.P[idx] : 0, k, z, h, T, Tn, flags, Mb, F.entry, rows, cols);
+		const int c = in.c, nbk = valid ? in.nbk : -1;
+		const unsigned fl = valid ? in.fl : 0u, fln = in.fln;
+		const double tcur = in.tcur, hcur = in.hcur, sz0 = in.sz0, zr = in.zr;
+		double hk = in.hk;
+		const bool mbc = in.mbc;
 		if (valid && !F.entry && writer) T[c] = tcur;
 		const bool member = valid && (fl & 1u);
 		const int vk = member ? vent_index(v, c) : -1;
@@ -575,6 +605,42 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 	}
 }
 
+// What a lane of sc_width_f reads of its cell c and of c's neighbour k
+// (the flows, T and the neighbours' h are not written by that launch).
+struct WidthIn {
+	int c, qk, vk, vn, vact, vnact;
+	double ih, it, in_k, out_k, nt_k;
+	bool mb;
+};
+
+__device__ __forceinline__ WidthIn width_load(int c, int k, const double* h, const double* __restrict__ T,
+                                              const double* fin, const double* fout,
+                                              const uint8_t* __restrict__ Mb, const int* vent_act,
+                                              const VentArgs& v, int rows, int cols)
+{
+	WidthIn r;
+	r.c = c;
+	const int i = c / cols, j = c - (c / cols) * cols;
+	r.ih = h[c];
+	r.it = T[c];
+	r.mb = Mb[c] != 0;
+	r.vk = vent_index(v, c);
+	r.vact = r.vk >= 0 ? vent_act[r.vk] : 0;
+	r.qk = nb_flat(i, j, k, rows, cols);
+	r.out_k = fout[c];
+	r.in_k = 0.0;
+	r.nt_k = 0.0;
+	r.vn = -1;
+	r.vnact = 0;
+	if (r.qk >= 0) {
+		r.in_k = fin[r.qk];
+		r.nt_k = T[r.qk];
+		r.vn = vent_index(v, r.qk);
+		r.vnact = r.vn >= 0 ? vent_act[r.vn] : 0;
+	}
+	return r;
+}
+
 __global__ __launch_bounds__(SC_BLOCK) void sc_width_f(double* __restrict__ z, double* __restrict__ h,
                                                        const double* __restrict__ T, double* __restrict__ Tn,
                                                        Flows8 f, const uint8_t* __restrict__ Mb,
@@ -582,14 +648,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width_f(double* __restrict__ z, d
                                                        FusedCtl F, VentArgs v, int rows, int cols, SciaraParams P)
 {
 	constexpr int opposite[NF] = {3, 2, 1, 0, 6, 7, 4, 5};
-	if (*F.stall) return;
 	const int tc = F.t % 3;
-	const int cnt = F.cnt[tc];
-	if (cnt == 0) {
-		// no member in P: the set is empty unless border cells were set aside
-		if (*F.skip == 0 && blockIdx.x == 0 && threadIdx.x == 0) *F.stall = F.t;
-		return;
-	}
 	const int lane = (int)(threadIdx.x & 63);
 	const int g0 = lane & ~7;
 	const int k = (lane & 7) + 1;
@@ -602,26 +661,35 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width_f(double* __restrict__ z, d
 			fin = f.f[opposite[q]];
 			fout = f.f[q];
 		}
+	// the first list entry and its loads go out before the list length and
+	// the stall word are known (an entry beyond the list is clamped into the
+	// grid and its values are not used)
+	const int ncell = rows * cols;
+	const size_t first = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3);
+	int c_pre = F.Lnew[min(first, (size_t)ncell - 1)];
+	c_pre = min(max(c_pre, 0), ncell - 1);
+	const WidthIn pre = width_load(c_pre, k, h, T, fin, fout, Mb, F.vent_act, v, rows, cols);
+	if (*F.stall) return;
+	const int cnt = F.cnt[tc];
+	if (cnt == 0) {
+		// no member in P: the set is empty unless border cells were set aside
+		if (*F.skip == 0 && blockIdx.x == 0 && threadIdx.x == 0) *F.stall = F.t;
+		return;
+	}
 	const size_t stride = ((size_t)gridDim.x * blockDim.x) >> 3;
-	for (size_t idx = (((size_t)blockIdx.x * blockDim.x) >> 3) + (threadIdx.x >> 3); idx < (size_t)cnt;
-	     idx += stride) {
-		const int c = F.Lnew[idx];
-		const int i = c / cols, j = c - (c / cols) * cols;
-		double ih = h[c], it = T[c];
-		const int vk = vent_index(v, c);
-		if (vk >= 0 && F.vent_act[vk] == F.t) {
-			ih = ih + v.e[vk];
+	for (size_t idx = first; idx < (size_t)cnt; idx += stride) {
+		const WidthIn in = idx == first ? pre : width_load(F.Lnew[idx], k, h, T, fin, fout, Mb, F.vent_act, v, rows, cols);
+		const int c = in.c;
+		double ih = in.ih, it = in.it;
+		if (in.vk >= 0 && in.vact == F.t) {
+			ih = ih + v.e[in.vk];
 			it = P.PTvent;
 		}
-		const int qk = nb_flat(i, j, k, rows, cols);
-		const double in_k = qk >= 0 ? fin[qk] : 0.0;
-		const double out_k = fout[c];
-		double nt_k = 0.0;
-		if (qk >= 0) {
-			nt_k = T[qk];
-			const int vn = vent_index(v, qk);
-			if (vn >= 0 && F.vent_act[vn] == F.t) nt_k = P.PTvent;
-		}
+		const int qk = in.qk;
+		const double in_k = in.in_k;
+		const double out_k = in.out_k;
+		double nt_k = in.nt_k;
+		if (qk >= 0 && in.vn >= 0 && in.vnact == F.t) nt_k = P.PTvent;
 		const double ht_k = in_k * nt_k;
 		double ht = 0.0, in_sum = 0.0, out_sum = 0.0;
 #pragma unroll
@@ -639,7 +707,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width_f(double* __restrict__ z, d
 			t1 = (residual * it + ht) / (residual + in_sum);
 		}
 		double h2 = h1, t2 = t1;
-		const bool mb = Mb[c] != 0;
+		const bool mb = in.mb;
 		if (h1 > 0 && !mb) {
 			const double nT = t1;
 			const double num = (((3.0 * cr_cube(nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
