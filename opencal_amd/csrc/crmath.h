@@ -299,4 +299,177 @@ CR_FN double cr_cos(double t)
 	return res.hi;
 }
 
+/* ---- first evaluations with Ziv's rounding test ----
+ *
+ * crz_pow10, crz_atan and crz_cos return the same double as cr_pow10,
+ * cr_atan and cr_cos, faster in nearly every case.  Each first evaluates
+ * its function as a double-double hi + lo with a relative error below about
+ * 2^-73 (10^x) and 2^-78 (atan, cos): the same argument reductions and
+ * tables as above, the leading terms of each series as exact products
+ * (fma) and the rest in double.  When every value within 2^-70 (relative)
+ * of hi + lo rounds to hi, hi is the correctly rounded result (Ziv, "Fast
+ * evaluation of elementary mathematical functions with correctly rounded
+ * last bit", ACM TOMS 17(3), 1991); otherwise, about once in 2^17 calls,
+ * the evaluation above decides.  tests/test_crmath.py compares them with
+ * cr_pow10, cr_atan and cr_cos over seeded samples of each domain. */
+
+/* hi = RN(hi + lo); 1 when every value within e of hi + lo rounds to hi */
+CR_FN int crz_rounds(double hi, double lo, double e) { return hi + (lo + e) == hi + (lo - e); }
+
+#define CRZ_BOUND 0x1p-70
+#define CRZ_THIRD_HI 0x1.5555555555555p-2   /* 1/3 = HI + LO */
+#define CRZ_THIRD_LO 0x1.5555555555555p-56
+#define CRZ_SIXTH_HI 0x1.5555555555555p-3   /* 1/6 = HI + LO */
+#define CRZ_SIXTH_LO 0x1.5555555555555p-57
+
+/* a (hi + lo) for a double-double and a constant khi + klo */
+CR_FN cr_dd_t crz_mul_k(cr_dd_t a, double khi, double klo)
+{
+	cr_dd_t p = cr_two_prod(a.hi, khi);
+	p.lo = fma(a.hi, klo, fma(a.lo, khi, p.lo));
+	return p;
+}
+
+CR_FN int crz_pow10_first(double x, double* out)
+{
+	if (!(x >= -300.0 && x <= 300.0)) return 0;
+	cr_dd_t X = cr_two_prod(x, CR_LN10_HI);
+	X.lo = fma(x, CR_LN10_LO, X.lo);
+	X = cr_fast_two_sum(X.hi, X.lo);
+	/* X = k ln2/64 + r as in cr_exp_core */
+	const double kd = floor(X.hi * CR_INV_LN2_64 + 0.5);
+	const int N = (int)kd;
+	const int j = N & 63;
+	const int m = (N - j) / 64;
+	const cr_dd_t p1 = cr_two_prod(kd, CR_LN2_64_1);
+	const cr_dd_t p2 = cr_two_prod(kd, CR_LN2_64_2);
+	const double r1 = X.hi - p1.hi;
+	const double s = ((X.lo - p1.lo) - p2.hi) - (p2.lo + kd * CR_LN2_64_3);
+	const cr_dd_t r = cr_two_sum(r1, s);
+	/* e^r - 1 = r + r^2/2 + r^3 (1/6 + r/24 + ... + r^5/8!), |r| <= 2^-7.4;
+	 * r.hi^2 exactly, r.lo through the square's cross term */
+	const cr_dd_t q = cr_two_prod(r.hi, r.hi);
+	double t = 1.0 / 40320.0;
+	t = t * r.hi + 1.0 / 5040.0;
+	t = t * r.hi + 1.0 / 720.0;
+	t = t * r.hi + 1.0 / 120.0;
+	t = t * r.hi + 1.0 / 24.0;
+	t = t * r.hi + 1.0 / 6.0;
+	const double tail3 = q.hi * (r.hi * t);
+	const cr_dd_t e0 = cr_two_sum(r.hi, 0.5 * q.hi);
+	const cr_dd_t em1 = cr_fast_two_sum(e0.hi, e0.lo + (r.lo + (0.5 * q.lo + r.hi * r.lo) + tail3));
+	/* 2^(j/64) (1 + em1) */
+	const cr_dd_t T = CR_EXP2_J64[j];
+	cr_dd_t p = cr_two_prod(T.hi, em1.hi);
+	p.lo = fma(T.hi, em1.lo, fma(T.lo, em1.hi, p.lo));
+	const cr_dd_t s2 = cr_two_sum(T.hi, p.hi);
+	const cr_dd_t res = cr_fast_two_sum(s2.hi, s2.lo + (T.lo + p.lo));
+	if (!crz_rounds(res.hi, res.lo, res.hi * CRZ_BOUND)) return 0;
+	*out = ldexp(res.hi, m);
+	return 1;
+}
+
+CR_FN int crz_atan_first(double x, double* out)
+{
+	const double ax = fabs(x);
+	if (!(ax >= 0x1p-27 && ax < 0x1p54)) return 0;
+	const int inv = ax > 1.0;
+	const double q0 = 1.0 / ax;
+	const double r0 = fma(-q0, ax, 1.0);
+	const cr_dd_t y = inv ? cr_fast_two_sum(q0, r0 * q0) : cr_mk(ax, 0.0);
+	const int j = (int)floor(y.hi * 64.0 + 0.5);
+	const double c = (double)j * 0.015625;
+	const cr_dd_t num = cr_two_sum(y.hi - c, y.lo);
+	cr_dd_t den = cr_two_prod(y.hi, c);
+	den.lo = fma(y.lo, c, den.lo);
+	den = cr_add_fast(cr_mk(1.0, 0.0), den);
+	/* u = num / den = uh + ul from one reciprocal and the remainder */
+	const double rd = 1.0 / den.hi;
+	const double uh = num.hi * rd;
+	const double ul = ((fma(-uh, den.hi, num.hi) + num.lo) - uh * den.lo) * rd;
+	/* atan u = u - u^3/3 + u^5 (1/5 - u^2/7 + ... + u^8/13), |u| <= 2^-7:
+	 * uh^3/3 from exact products, ul through the cubic's derivative */
+	const cr_dd_t v = cr_two_prod(uh, uh);
+	cr_dd_t w = cr_two_prod(uh, v.hi);
+	w.lo = fma(uh, v.lo, w.lo);
+	const cr_dd_t c3 = crz_mul_k(w, -CRZ_THIRD_HI, -CRZ_THIRD_LO);
+	double t = 1.0 / 13.0;
+	t = t * v.hi - 1.0 / 11.0;
+	t = t * v.hi + 1.0 / 9.0;
+	t = t * v.hi - 1.0 / 7.0;
+	t = t * v.hi + 1.0 / 5.0;
+	const double h5 = w.hi * (v.hi * t);
+	const cr_dd_t T = CR_ATAN_J64[j];
+	const cr_dd_t s1 = cr_two_sum(T.hi, uh);
+	const cr_dd_t s2 = cr_two_sum(s1.hi, c3.hi);
+	const double lo = (s1.lo + s2.lo) + (T.lo + ((ul - v.hi * ul) + (c3.lo + h5)));
+	cr_dd_t res = cr_fast_two_sum(s2.hi, lo);
+	if (inv) res = cr_add(cr_mk(CR_PI_2_HI, CR_PI_2_MID), cr_neg(res));
+	if (!crz_rounds(res.hi, res.lo, res.hi * CRZ_BOUND)) return 0;
+	*out = x < 0.0 ? -res.hi : res.hi;
+	return 1;
+}
+
+CR_FN int crz_cos_first(double t, double* out)
+{
+	const double a = fabs(t);
+	if (!(a >= 0x1p-27 && a <= CR_PI_2_HI + 0.0078125)) return 0;
+	const int sine = a > CR_PI_4;
+	const cr_dd_t w = sine ? cr_add(cr_two_sum(CR_PI_2_HI - a, CR_PI_2_MID), cr_mk(CR_PI_2_LO, 0.0))
+	                       : cr_mk(a, 0.0);
+	const int j = (int)floor(w.hi * 64.0 + 0.5);
+	const double c = (double)j * 0.015625;
+	const cr_dd_t d = cr_two_sum(w.hi - c, w.lo); /* |d| <= 2^-7 */
+	const cr_dd_t v = cr_two_prod(d.hi, d.hi);
+	/* sin d = d - d^3/6 + d^5 (1/5! - d^2/7! + d^4/9!): d.hi^3/6 from
+	 * exact products, d.lo through the derivative 1 - d^2/2 */
+	cr_dd_t d3 = cr_two_prod(d.hi, v.hi);
+	d3.lo = fma(d.hi, v.lo, d3.lo);
+	const cr_dd_t c3 = crz_mul_k(d3, -CRZ_SIXTH_HI, -CRZ_SIXTH_LO);
+	double ts = 1.0 / 362880.0;
+	ts = ts * v.hi - 1.0 / 5040.0;
+	ts = ts * v.hi + 1.0 / 120.0;
+	const double h5 = d3.hi * (v.hi * ts);
+	const cr_dd_t s0 = cr_two_sum(d.hi, c3.hi);
+	const cr_dd_t S = cr_fast_two_sum(s0.hi, s0.lo + ((d.lo - 0.5 * v.hi * d.lo) + (c3.lo + h5)));
+	/* cos d - 1 = -d^2/2 + d^4 (1/4! - d^2/6! + d^4/8!) */
+	double tc = 1.0 / 40320.0;
+	tc = tc * v.hi - 1.0 / 720.0;
+	tc = tc * v.hi + 1.0 / 24.0;
+	const double h4 = v.hi * (v.hi * tc);
+	const cr_dd_t CM = cr_fast_two_sum(-0.5 * v.hi, (-0.5 * v.lo - d.hi * d.lo) + h4);
+	/* cos(c + d) = cos c + (cos c (cos d - 1) - sin c sin d)
+	 * sin(c + d) = sin c + (sin c (cos d - 1) + cos c sin d) */
+	const cr_dd_t A = sine ? CR_SIN_J64[j] : CR_COS_J64[j];
+	const cr_dd_t B = sine ? CR_COS_J64[j] : cr_neg(CR_SIN_J64[j]);
+	cr_dd_t P1 = cr_two_prod(A.hi, CM.hi);
+	P1.lo = fma(A.hi, CM.lo, fma(A.lo, CM.hi, P1.lo));
+	cr_dd_t P2 = cr_two_prod(B.hi, S.hi);
+	P2.lo = fma(B.hi, S.lo, fma(B.lo, S.hi, P2.lo));
+	const cr_dd_t s1 = cr_two_sum(A.hi, P2.hi);
+	const cr_dd_t s2 = cr_two_sum(s1.hi, P1.hi);
+	const cr_dd_t res = cr_fast_two_sum(s2.hi, (s1.lo + s2.lo) + (A.lo + (P1.lo + P2.lo)));
+	if (!crz_rounds(res.hi, res.lo, res.hi * CRZ_BOUND)) return 0;
+	*out = res.hi;
+	return 1;
+}
+
+CR_FN double crz_pow10(double x)
+{
+	double r;
+	return crz_pow10_first(x, &r) ? r : cr_pow10(x);
+}
+
+CR_FN double crz_atan(double x)
+{
+	double r;
+	return crz_atan_first(x, &r) ? r : cr_atan(x);
+}
+
+CR_FN double crz_cos(double t)
+{
+	double r;
+	return crz_cos_first(t, &r) ? r : cr_cos(t);
+}
+
 #endif

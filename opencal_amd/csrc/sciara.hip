@@ -26,10 +26,11 @@
 //   count[0] list length, count[1] cells appended by sc_flows,
 //   count[2] survivors written by sc_finish, count[3] finished blocks (the
 //   last block moves count[2] to count[0] and clears the others).
-// Elementary functions are the correctly rounded ones of crmath.h; the
-// oracle built with the same source reproduces every bit, and a glibc build
-// differs only where glibc itself rounds the wrong way (about one call in
-// 10^3 to 10^4, by one ulp).
+// Elementary functions are the correctly rounded ones of crmath.h (10^x,
+// atan and cos through their first evaluation with Ziv's rounding test,
+// the same bits as the full one); the oracle built with the same source
+// reproduces every bit, and a glibc build differs only where glibc itself
+// rounds the wrong way (about one call in 10^3 to 10^4, by one ulp).
 #include <math.h>
 #include <string.h>
 
@@ -155,8 +156,8 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 		if (!(h0 > 0)) continue;
 		const int i = c / cols, j = c - (c / cols) * cols;
 		const double t = T[c];
-		const double Pr = cr_pow10(P.a + P.b * t);
-		const double hc = cr_pow10(P.c + P.d * t);
+		const double Pr = crz_pow10(P.a + P.b * t);
+		const double hc = crz_pow10(P.c + P.d * t);
 		const double sz0 = z[c];
 		const int nbk = nb_flat(i, j, k, rows, cols);
 		const double hk = nbk >= 0 ? h[nbk] : 0.0;
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows(const double* __restrict__ 
 		const double lk = zk + hk;
 		const bool down = lvl0 > lk;
 		const double Hk = down ? lk : 0.0;
-		const double hcck = hc * cr_cos(cr_atan((lvl0 - lk) / P.Pc));
+		const double hcck = hc * crz_cos(crz_atan((lvl0 - lk) / P.Pc));
 		double H[9];
 		H[0] = sz0;
 #pragma unroll
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 		}
 		if (member && !wet) f.f[k - 1][c] = 0.0;
 		if (wet) {
-			const double pw = cr_pow10((lane & 1) ? P.c + P.d * t : P.a + P.b * t);
+			const double pw = crz_pow10((lane & 1) ? P.c + P.d * t : P.a + P.b * t);
 			const double Pr = __shfl(pw, g0, 64);
 			const double hc = __shfl(pw, g0 + 1, 64);
 			if (nbk >= 0) {
@@ -559,7 +560,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 			const double lk = zk + hk;
 			const bool down = lvl0 > lk;
 			const double Hk = down ? lk : 0.0;
-			const double hcck = hc * cr_cos(cr_atan((lvl0 - lk) / P.Pc));
+			const double hcck = hc * crz_cos(crz_atan((lvl0 - lk) / P.Pc));
 			double H[9];
 			H[0] = sz0;
 #pragma unroll

@@ -19,6 +19,7 @@
 #include <math.h>
 #include <quadmath.h>
 #include <stdint.h>
+#include <string.h>
 
 static double cr_fn(int fn, double x)
 {
@@ -133,5 +134,36 @@ int orc_cr_check(int fn, double lo, double hi, int log_scale, long n, unsigned l
 	out[2] = max_l;
 	out[3] = max_q;
 	out[4] = (double)bad_lq;
+	return 0;
+}
+
+/* The first evaluations with Ziv's rounding test (crz_pow10, crz_atan,
+ * crz_cos of crmath.h, which the GPU runs) against the full evaluations
+ * (cr_pow10, cr_atan, cr_cos), over n seeded points of a domain: out[0]
+ * points where the two results differ (0 expected), out[1] points where the
+ * first evaluation left the result to the full one.  fn: 0 pow(10, x),
+ * 1 atan, 2 cos. */
+int orc_crz_check(int fn, double lo, double hi, int log_scale, long n, unsigned long long seed, double* out)
+{
+	long k, bad = 0, slow = 0;
+	if (fn < 0 || fn > 2) return -1;
+#pragma omp parallel for reduction(+ : bad, slow) schedule(static)
+	for (k = 0; k < n; k++) {
+		const double x = point(lo, hi, log_scale, seed, k);
+		double f = 0.0;
+		int ok = 0;
+		double z;
+		switch (fn) {
+		case 0: ok = crz_pow10_first(x, &f); z = crz_pow10(x); break;
+		case 1: ok = crz_atan_first(x, &f); z = crz_atan(x); break;
+		default: ok = crz_cos_first(x, &f); z = crz_cos(x); break;
+		}
+		const double c = cr_fn(fn, x);
+		if (!ok) slow++;
+		/* bit patterns: a NaN result must be the same NaN */
+		if (memcmp(&z, &c, sizeof z) != 0 || (ok && memcmp(&f, &c, sizeof f) != 0)) bad++;
+	}
+	out[0] = (double)bad;
+	out[1] = (double)slow;
 	return 0;
 }
