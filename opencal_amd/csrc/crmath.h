@@ -301,17 +301,17 @@ CR_FN double cr_cos(double t)
 
 /* ---- first evaluations with Ziv's rounding test ----
  *
- * crz_pow10, crz_atan and crz_cos return the same double as cr_pow10,
- * cr_atan and cr_cos, faster in nearly every case.  Each first evaluates
- * its function as a double-double hi + lo with a relative error below about
- * 2^-73 (10^x) and 2^-78 (atan, cos): the same argument reductions and
+ * crz_pow10, crz_atan, crz_cos and crz_pow_third return the same double as
+ * cr_pow10, cr_atan, cr_cos and cr_pow_third, faster in nearly every case.
+ * Each first evaluates its function as a double-double hi + lo with a
+ * relative error below about 2^-73 (10^x, x^(1/3)) and 2^-78 (atan, cos): the same argument reductions and
  * tables as above, the leading terms of each series as exact products
  * (fma) and the rest in double.  When every value within 2^-70 (relative)
  * of hi + lo rounds to hi, hi is the correctly rounded result (Ziv, "Fast
  * evaluation of elementary mathematical functions with correctly rounded
  * last bit", ACM TOMS 17(3), 1991); otherwise, about once in 2^17 calls,
  * the evaluation above decides.  tests/test_crmath.py compares them with
- * cr_pow10, cr_atan and cr_cos over seeded samples of each domain. */
+ * the full evaluations over seeded samples of each domain. */
 
 /* hi = RN(hi + lo); 1 when every value within e of hi + lo rounds to hi */
 CR_FN int crz_rounds(double hi, double lo, double e) { return hi + (lo + e) == hi + (lo - e); }
@@ -330,12 +330,9 @@ CR_FN cr_dd_t crz_mul_k(cr_dd_t a, double khi, double klo)
 	return p;
 }
 
-CR_FN int crz_pow10_first(double x, double* out)
+/* e^X = 2^m (hi + lo) for |X.hi| <= 700, relative error below 2^-73 */
+CR_FN cr_dd_t crz_exp_first(cr_dd_t X, int* mout)
 {
-	if (!(x >= -300.0 && x <= 300.0)) return 0;
-	cr_dd_t X = cr_two_prod(x, CR_LN10_HI);
-	X.lo = fma(x, CR_LN10_LO, X.lo);
-	X = cr_fast_two_sum(X.hi, X.lo);
 	/* X = k ln2/64 + r as in cr_exp_core */
 	const double kd = floor(X.hi * CR_INV_LN2_64 + 0.5);
 	const int N = (int)kd;
@@ -363,9 +360,61 @@ CR_FN int crz_pow10_first(double x, double* out)
 	cr_dd_t p = cr_two_prod(T.hi, em1.hi);
 	p.lo = fma(T.hi, em1.lo, fma(T.lo, em1.hi, p.lo));
 	const cr_dd_t s2 = cr_two_sum(T.hi, p.hi);
-	const cr_dd_t res = cr_fast_two_sum(s2.hi, s2.lo + (T.lo + p.lo));
+	*mout = m;
+	return cr_fast_two_sum(s2.hi, s2.lo + (T.lo + p.lo));
+}
+
+CR_FN int crz_pow10_first(double x, double* out)
+{
+	if (!(x >= -300.0 && x <= 300.0)) return 0;
+	cr_dd_t X = cr_two_prod(x, CR_LN10_HI);
+	X.lo = fma(x, CR_LN10_LO, X.lo);
+	X = cr_fast_two_sum(X.hi, X.lo);
+	int m;
+	const cr_dd_t res = crz_exp_first(X, &m);
 	if (!crz_rounds(res.hi, res.lo, res.hi * CRZ_BOUND)) return 0;
 	*out = ldexp(res.hi, m);
+	return 1;
+}
+
+/* pow(x, 1.0 / 3.0) for 2^-1000 <= x <= 2^1000: ln x as cr_log_core reduces
+ * it, with 2 atanh(s) = 2 (s + s^3/3 + s^5 (1/5 + ... + s^8/13)), |s| <=
+ * 2^-8, s^3/3 from exact products; then e^(ln x RN(1/3)), relative error
+ * below 2^-73 */
+CR_FN int crz_pow_third_first(double x, double* out)
+{
+	if (!(x >= 0x1p-1000 && x <= 0x1p1000)) return 0;
+	int e;
+	double m = frexp(x, &e); /* [0.5, 1) */
+	m = m + m;
+	e = e - 1;
+	const int j = (int)floor((m - 1.0) * 64.0 + 0.5); /* 0..64 */
+	const double c = 1.0 + (double)j * 0.015625;
+	const double num = m - c; /* exact (Sterbenz) */
+	const cr_dd_t den = cr_two_sum(m, c);
+	const double rd = 1.0 / den.hi;
+	const double sh = num * rd;
+	const double sl = (fma(-sh, den.hi, num) - sh * den.lo) * rd;
+	const cr_dd_t v = cr_two_prod(sh, sh);
+	cr_dd_t w = cr_two_prod(sh, v.hi);
+	w.lo = fma(sh, v.lo, w.lo);
+	const cr_dd_t c3 = crz_mul_k(w, CRZ_THIRD_HI, CRZ_THIRD_LO);
+	double t = 1.0 / 13.0;
+	t = t * v.hi + 1.0 / 11.0;
+	t = t * v.hi + 1.0 / 9.0;
+	t = t * v.hi + 1.0 / 7.0;
+	t = t * v.hi + 1.0 / 5.0;
+	const double h5 = w.hi * (v.hi * t);
+	const cr_dd_t a0 = cr_two_sum(sh, c3.hi);
+	const cr_dd_t at = cr_fast_two_sum(a0.hi, a0.lo + ((sl + v.hi * sl) + (c3.lo + h5)));
+	cr_dd_t E = cr_two_prod((double)e, CR_LN2_HI);
+	E.lo = fma((double)e, CR_LN2_LO, E.lo);
+	E = cr_fast_two_sum(E.hi, E.lo);
+	const cr_dd_t L = cr_add(cr_add(E, CR_LOG_C64[j]), cr_mk(2.0 * at.hi, 2.0 * at.lo));
+	int mm;
+	const cr_dd_t res = crz_exp_first(cr_mul_d(L, 1.0 / 3.0), &mm);
+	if (!crz_rounds(res.hi, res.lo, res.hi * CRZ_BOUND)) return 0;
+	*out = ldexp(res.hi, mm);
 	return 1;
 }
 
@@ -470,6 +519,12 @@ CR_FN double crz_cos(double t)
 {
 	double r;
 	return crz_cos_first(t, &r) ? r : cr_cos(t);
+}
+
+CR_FN double crz_pow_third(double x)
+{
+	double r;
+	return crz_pow_third_first(x, &r) ? r : cr_pow_third(x);
 }
 
 #endif

@@ -268,7 +268,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width(double* __restrict__ z, dou
 			const double num = (((3.0 * cr_cube(nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
 			const double den = ((P.Prho * P.Pcv) * h1) * P.Pac;
 			const double aus = 1.0 + num / den;
-			const double st = nT / cr_pow_third(aus);
+			const double st = nT / crz_pow_third(aus);
 			t2 = st;
 			if (st <= P.PTsol) {
 				if (writer) {
@@ -592,12 +592,22 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_flows_f(const double* __restrict_
 			// writes the positive ones (Sciara.cpp:95-101)
 			f.f[k - 1][c] = fk > 0 ? fk : 0.0;
 			// a receiver outside the set joins the next list once (flag bit
-			// 1); the set's cells are listed below
+			// 1), its slot from one atomic per wave; the set's cells are
+			// listed below
+			bool fresh = false;
 			if (fk > 0 && nbk >= 0 && !(fln & 1u)) {
 				unsigned* w = reinterpret_cast<unsigned*>(flags + (nbk & ~3));
 				const int sh = (nbk & 3) * 8;
 				const unsigned old = atomicOr(w, 2u << sh);
-				if (((old >> sh) & 3u) == 0u) F.Lnew[atomicAdd(F.cnt + tc, 1)] = nbk;
+				fresh = ((old >> sh) & 3u) == 0u;
+			}
+			const unsigned long long rm = __ballot(fresh);
+			if (rm) {
+				const int leader = __ffsll((long long)rm) - 1;
+				int rb = 0;
+				if (lane == leader) rb = atomicAdd(F.cnt + tc, __popcll(rm));
+				rb = __shfl(rb, leader, 64);
+				if (fresh) F.Lnew[rb + __popcll(rm & below)] = nbk;
 			}
 		}
 		if (threadIdx.x == 0) s_base[pass] = base;
@@ -714,7 +724,7 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_width_f(double* __restrict__ z, d
 			const double num = (((3.0 * cr_cube(nT)) * P.Peps) * P.Psigma * P.Pclock) * P.Pcool;
 			const double den = ((P.Prho * P.Pcv) * h1) * P.Pac;
 			const double aus = 1.0 + num / den;
-			const double st = nT / cr_pow_third(aus);
+			const double st = nT / crz_pow_third(aus);
 			t2 = st;
 			if (st <= P.PTsol) {
 				if (writer) {

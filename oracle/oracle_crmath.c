@@ -138,15 +138,15 @@ int orc_cr_check(int fn, double lo, double hi, int log_scale, long n, unsigned l
 }
 
 /* The first evaluations with Ziv's rounding test (crz_pow10, crz_atan,
- * crz_cos of crmath.h, which the GPU runs) against the full evaluations
- * (cr_pow10, cr_atan, cr_cos), over n seeded points of a domain: out[0]
- * points where the two results differ (0 expected), out[1] points where the
- * first evaluation left the result to the full one.  fn: 0 pow(10, x),
- * 1 atan, 2 cos. */
+ * crz_cos, crz_pow_third of crmath.h, which the GPU runs) against the full
+ * evaluations (cr_pow10, cr_atan, cr_cos, cr_pow_third), over n seeded
+ * points of a domain: out[0] points where the two results differ (0
+ * expected), out[1] points where the first evaluation left the result to
+ * the full one.  fn: 0 pow(10, x), 1 atan, 2 cos, 4 pow(x, 1.0/3.0). */
 int orc_crz_check(int fn, double lo, double hi, int log_scale, long n, unsigned long long seed, double* out)
 {
 	long k, bad = 0, slow = 0;
-	if (fn < 0 || fn > 2) return -1;
+	if (fn < 0 || fn > 4 || fn == 3) return -1;
 #pragma omp parallel for reduction(+ : bad, slow) schedule(static)
 	for (k = 0; k < n; k++) {
 		const double x = point(lo, hi, log_scale, seed, k);
@@ -156,7 +156,8 @@ int orc_crz_check(int fn, double lo, double hi, int log_scale, long n, unsigned 
 		switch (fn) {
 		case 0: ok = crz_pow10_first(x, &f); z = crz_pow10(x); break;
 		case 1: ok = crz_atan_first(x, &f); z = crz_atan(x); break;
-		default: ok = crz_cos_first(x, &f); z = crz_cos(x); break;
+		case 2: ok = crz_cos_first(x, &f); z = crz_cos(x); break;
+		default: ok = crz_pow_third_first(x, &f); z = crz_pow_third(x); break;
 		}
 		const double c = cr_fn(fn, x);
 		if (!ok) slow++;

@@ -106,22 +106,25 @@ def test_values_where_glibc_misrounds_are_binary128():
     assert (np.abs(c[diff] - m[diff]) <= np.spacing(np.abs(m[diff]))).all()
 
 
+# (fn, lo, hi, log scale, whole domain inside the first evaluation's range)
 CRZ_DOMAINS = [
-    (0, -1.2, 2.0, 0), (0, -300.0, 300.0, 0), (0, -330.0, 308.5, 0),
-    (1, 1e-8, 1e8, 1), (1, -3.0, 3.0, 0), (1, 1e-40, 1e40, 1), (1, -500.0, 500.0, 0),
-    (2, 0.0, 1.5786, 0), (2, 1.5, 1.5786, 0), (2, 1e-12, 0.8, 1), (2, -1.6, 1.6, 0),
+    (0, -1.2, 2.0, 0, 1), (0, -300.0, 300.0, 0, 1), (0, -330.0, 308.5, 0, 0),
+    (1, 1e-8, 1e8, 1, 1), (1, -3.0, 3.0, 0, 1), (1, 1e-40, 1e40, 1, 0), (1, -500.0, 500.0, 0, 1),
+    (2, 0.0, 1.5786, 0, 1), (2, 1.5, 1.5786, 0, 1), (2, 1e-12, 0.8, 1, 0), (2, -1.6, 1.6, 0, 0),
+    (4, 1.0, 1.001, 0, 1), (4, 0.5, 2.0, 0, 1), (4, 1e-300, 1e300, 1, 1), (4, 1e-320, 1e-290, 1, 0),
 ]
 
 
 @pytest.mark.parametrize("dom", CRZ_DOMAINS, ids=["%d:%g..%g" % d[:3] for d in CRZ_DOMAINS])
 def test_first_evaluation_equals_full(dom):
-    """crz_pow10 / crz_atan / crz_cos (a cheaper first evaluation, Ziv's
-    rounding test, the full evaluation when the test fails; what the GPU
-    runs) return the bits of cr_pow10 / cr_atan / cr_cos at every point,
+    """crz_pow10 / crz_atan / crz_cos / crz_pow_third (a cheaper first
+    evaluation, Ziv's rounding test, the full evaluation when the test
+    fails; what the GPU runs) return the bits of cr_pow10 / cr_atan / cr_cos
+    / cr_pow_third at every point,
     NaN and the cases outside the first evaluation's range included; inside
     that range the full evaluation is needed at about 2^-16 of the points.
-    (A sweep of 8 x 10^8 points over these domains found no difference.)"""
-    fn, lo, hi, lg = dom
+    (A sweep of 1.3 x 10^9 points over these domains found no difference.)"""
+    fn, lo, hi, lg, inside = dom
     lib = native.oracle()
     lib.orc_crz_check.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_long, C.c_ulonglong,
                                   C.POINTER(C.c_double)]
@@ -129,5 +132,5 @@ def test_first_evaluation_equals_full(dom):
     n = 4_000_000
     assert lib.orc_crz_check(fn, lo, hi, lg, n, 1234 + fn, out) == 0
     assert out[0] == 0
-    if lg == 0 and (fn != 2 or hi < 1.58) and lo > -301:
+    if inside:
         assert out[1] < n * 2.0 ** -13   # rounding-test failures only
