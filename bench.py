@@ -7,10 +7,10 @@ transition, the model's steering — the reference flow reset, declared with
 calhipBindSciddicaTSteering2D so that its device twin runs — and the implicit
 update), with the model already resident in HBM.  Between two steps the
 flows are reset, so a step is a function of z and h alone and the engine
-advances the grid up to eight steps per launch: the tiles within reach of a
+advances the grid up to nine steps per launch: the tiles within reach of a
 wet cell in LDS, every other cell by a streaming copy, each launch reading
 and writing every cell once.  The same workload with one calRunCAStep2D
-call per step (which the engine groups eight to a launch) is reported
+call per step (which the engine groups nine to a launch) is reported
 beside it (`per_step_driver`).  Unit: giga cell updates per second.
 
 The same JSON line carries
@@ -590,13 +590,13 @@ def main():
         "config": {"workload": f"sciddicaT_plain_{args.size}x{args.size}_fp64", "rows": args.size,
                    "cols": args.size, "p_r": 0.5, "p_epsilon": 0.001, "optimization": "CAL_NO_OPT",
                    "driver": "calRun2D's step loop (calhipRunSteps2D), steering = the reference flow reset "
-                             "with its device twin: up to eight steps per launch"},
+                             "with its device twin: up to nine steps per launch"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic() if args.size == 4096 else None,
                      "traffic_source": "profiles/sciddicat_tiles_4096x4096.md (rocprofv3 FETCH_SIZE x 2.0 + "
-                                       "WRITE_SIZE, per 8-step launch)",
-                     "kernel": "plain_tiles<8>", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
+                                       "WRITE_SIZE, per 9-step launch)",
+                     "kernel": "plain_tiles<9>", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
                      "steps_per_launch": round(r["steps_per_launch"], 4),
                      "algorithmic_bytes_per_launch": int(MIN_BYTES_PER_CELL_SCIDDICAT * args.size * args.size),
                      "note": "achieved = the bytes one launch must move (h read once and written once: 16 B "
@@ -607,14 +607,14 @@ def main():
                              "traffic = measured HBM bytes per launch (rocprofv3 counters), traffic_gbs / "
                              "traffic_frac over the same time"},
         "per_step_driver": {
-            "driver": "calRunCAStep2D per step (the engine groups eight steps to a launch)",
+            "driver": "calRunCAStep2D per step (the engine groups nine steps to a launch)",
             "gcups": round(r1["gcups"], 3),
             "ms_per_step": round(r1["wall_s"] / args.steps * 1e3, 4),
             "roofline": {"bound": "hbm", "achieved": round(r1["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(r1["achieved_gbs"] / HBM_PEAK_GBS, 4),
                          "traffic": profiled_traffic() if args.size == 4096 else None,
                          "traffic_source": "profiles/sciddicat_tiles_4096x4096.md",
-                         "kernel": "plain_tiles<8>", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2),
+                         "kernel": "plain_tiles<9>", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2),
                          "steps_per_launch": round(r1["steps_per_launch"], 4),
                          "algorithmic_bytes_per_launch": int(MIN_BYTES_PER_CELL_SCIDDICAT * args.size * args.size)}},
     }

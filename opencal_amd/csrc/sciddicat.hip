@@ -494,7 +494,7 @@ struct SciddicaTProgram : Program {
 	// The full grid in launches of up to PL_KMAX steps over the same tiles
 	// (sciddicat_active.hpp), for steps followed by the steering twin while
 	// few tiles are near wet cells.
-	static constexpr int PL_KMAX = 8;
+	static constexpr int PL_KMAX = actf::PLAIN_KMAX;
 	struct PlainBufs {
 		unsigned* tag[2] = {nullptr, nullptr};
 		int* list[2] = {nullptr, nullptr};
@@ -863,7 +863,7 @@ struct SciddicaTProgram : Program {
 		return p;
 	}
 
-	static int pl_pick(int lim) { return lim >= 8 ? 8 : (lim >= 4 ? 4 : (lim >= 2 ? 2 : 1)); }
+	static int pl_pick(int lim) { return std::max(1, std::min(lim, PL_KMAX)); }
 
 	// Up to n steps of the full grid (each followed by the steering twin);
 	// returns the steps taken: fewer than n when the mode stops paying (too
@@ -886,7 +886,8 @@ struct SciddicaTProgram : Program {
 			const int Kn = pl_pick(n - done > K ? std::min(n - done - K, pl.km) : pl.km);
 			actf::PlainLaunch p = pl_args(m);
 			p.km = Kn;
-			p.grid_comp = std::max(1, std::min(pl.n_tiles, pl.cus * (K == 8 ? 1 : 2)));
+			// compute blocks: as many as fit beside each other (LDS: one per CU from K = 8)
+			p.grid_comp = std::max(1, std::min(pl.n_tiles, pl.cus * (K >= 8 ? 1 : 2)));
 			hipEvent_t ev;
 			timer_begin(m, &ev);
 			if (actf::launch_plain(m->stream, K, p)) return -1;

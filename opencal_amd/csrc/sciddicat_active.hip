@@ -354,7 +354,10 @@ __global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
 	constexpr int H = 2 * K, RW = S::RW, NC = S::NC, CPT = S::CPT, NT = S::NT;
 	__shared__ double su[NC];
 	__shared__ double so[4][NC];
-	__shared__ unsigned sh_mark;
+	// the tile marks live in so[0][0]: outflows are written for rows and
+	// columns 1 .. RW - 2 and read for 1 .. RW - 2 only, so the corner entry
+	// is never a flow
+	unsigned& sh_mark = *reinterpret_cast<unsigned*>(&so[0][0]);
 	const int tid = threadIdx.x;
 	if ((int)blockIdx.x >= p.grid_comp) {
 		// copy role
@@ -519,7 +522,17 @@ int launch_leave(hipStream_t st, double* h, const uint8_t* fl, const uint8_t* gh
 
 int plain_threads(int K)
 {
-	return K == 8 ? PlainShape<8>::NT : (K == 4 ? PlainShape<4>::NT : (K == 2 ? PlainShape<2>::NT : PlainShape<1>::NT));
+	switch (K) {
+	case 1: return PlainShape<1>::NT;
+	case 2: return PlainShape<2>::NT;
+	case 3: return PlainShape<3>::NT;
+	case 4: return PlainShape<4>::NT;
+	case 5: return PlainShape<5>::NT;
+	case 6: return PlainShape<6>::NT;
+	case 7: return PlainShape<7>::NT;
+	case 8: return PlainShape<8>::NT;
+	default: return PlainShape<9>::NT;
+	}
 }
 
 int plain_copy_blocks(int K, int rows, int cols)
@@ -535,25 +548,29 @@ int launch_plain_setup(hipStream_t st, const PlainLaunch& p)
 	return CALHIP_CHECK(hipGetLastError());
 }
 
+template <int K>
+static void plain_go(hipStream_t st, int grid, const PlainLaunch& p)
+{
+	hipLaunchKernelGGL(plain_tiles<K>, dim3(grid), dim3(PlainShape<K>::NT), 0, st, p);
+}
+
 int launch_plain(hipStream_t st, int K, const PlainLaunch& p)
 {
+	if (K < 1 || K > PLAIN_KMAX) {
+		set_error(13, "SciddicaT full-grid launch: steps per launch must be 1 to 9");
+		return -1;
+	}
 	const int grid = p.grid_comp + plain_copy_blocks(K, p.rows, p.cols);
 	switch (K) {
-	case 1:
-		hipLaunchKernelGGL(plain_tiles<1>, dim3(grid), dim3(PlainShape<1>::NT), 0, st, p);
-		break;
-	case 2:
-		hipLaunchKernelGGL(plain_tiles<2>, dim3(grid), dim3(PlainShape<2>::NT), 0, st, p);
-		break;
-	case 4:
-		hipLaunchKernelGGL(plain_tiles<4>, dim3(grid), dim3(PlainShape<4>::NT), 0, st, p);
-		break;
-	case 8:
-		hipLaunchKernelGGL(plain_tiles<8>, dim3(grid), dim3(PlainShape<8>::NT), 0, st, p);
-		break;
-	default:
-		set_error(13, "SciddicaT full-grid launch: steps per launch must be 1, 2, 4 or 8");
-		return -1;
+	case 1: plain_go<1>(st, grid, p); break;
+	case 2: plain_go<2>(st, grid, p); break;
+	case 3: plain_go<3>(st, grid, p); break;
+	case 4: plain_go<4>(st, grid, p); break;
+	case 5: plain_go<5>(st, grid, p); break;
+	case 6: plain_go<6>(st, grid, p); break;
+	case 7: plain_go<7>(st, grid, p); break;
+	case 8: plain_go<8>(st, grid, p); break;
+	default: plain_go<9>(st, grid, p); break;
 	}
 	return CALHIP_CHECK(hipGetLastError());
 }

@@ -124,6 +124,8 @@ struct PlainLaunch {
 	double r, eps;
 };
 
+constexpr int PLAIN_KMAX = 9;   // most steps per full-grid launch (1 .. 9)
+
 // threads per block of the K-step kernel and the column width of one copy block
 int plain_threads(int K);
 int plain_copy_blocks(int K, int rows, int cols);

@@ -1,8 +1,8 @@
-"""SciddicaT on the full grid (CAL_NO_OPT) in launches of up to 8 steps: the
+"""SciddicaT on the full grid (CAL_NO_OPT) in launches of up to 9 steps: the
 tiles near wet cells advance in LDS, every other tile is copied (h + 0.0),
 each launch reading and writing every cell once.  Checked bit for bit
 against the oracle (zero signs included) through calRun2D's loop, one
-calRunCAStep2D per step (grouped eight to a launch) and mixtures, on
+calRunCAStep2D per step (grouped nine to a launch) and mixtures, on
 tile-ragged grid shapes, with special values, and with the mode forced on
 (CALHIP_SCIDDICAT_TILES_ALWAYS) where the wet area would otherwise make the
 engine keep the strip sweeps."""
@@ -51,7 +51,7 @@ ALWAYS = {"CALHIP_SCIDDICAT_TILES_ALWAYS": "1"}
 
 
 @pytest.mark.parametrize("rows,cols,steps,seed", [
-    (64, 64, 61, 1),      # whole tiles: 8 steps per launch
+    (64, 64, 61, 1),      # whole tiles: 9 steps per launch
     (37, 53, 45, 2),      # last tile row and column of 5 cells: up to 4 steps per launch
     (100, 70, 33, 3),
     (17, 17, 40, 4),      # two tile rows and columns, the second of one cell
@@ -74,13 +74,13 @@ def test_plain_tiles_vs_oracle(rows, cols, steps, seed):
 
 def test_plain_tiles_launches():
     """At 256 x 256 with a 12 x 12 source the mode turns on by itself and
-    advances eight steps per launch."""
+    advances nine steps per launch."""
     z, h, _ = synth.sciddicat_inputs(256, 256, 10, patch=12)
     _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 81, False)
     gh, launches, _ = run_plan(z, h, 0.5, 0.001, [("run", 81)])
     print(f"launches {launches}", flush=True)
     assert same(gh, oh)
-    assert launches <= 1 + 10 + 3   # entry list, 80 / 8 launches, the first step's own launches
+    assert launches <= 1 + 9 + 3   # entry list, 80 steps in 9 launches (8 x 9 + 8), the first step's own
 
 
 @pytest.mark.parametrize("p_r,p_eps", [(0.3, 0.0), (0.9, 1e-2), (1.0, 1e-3)])
