@@ -370,16 +370,25 @@ __global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
 		const int tj0 = j / TILE, tj1 = (j + 1) / TILE;
 		const bool c0 = p.tag_in[ti + tj0 * p.ty] != p.e;
 		const bool c1 = j + 1 < p.cols && p.tag_in[ti + tj1 * p.ty] != p.e;
-		const int i1 = min(p.rows, (ti + 1) * TILE);
+		const int i0 = ti * TILE, nr = min(TILE, p.rows - i0);
 		const bool pair = c0 && c1 && (p.cols % 2 == 0);   // j even: 16-byte aligned pair
-		for (int i = ti * TILE; i < i1; ++i) {
-			const size_t c = (size_t)i * p.cols + j;
-			if (pair) {
-				double2 v = *reinterpret_cast<const double2*>(p.h_in + c);
-				v.x += 0.0;   // h + (0.0 - 0.0) x 4: every flow around the cell is zero
-				v.y += 0.0;
-				*reinterpret_cast<double2*>(p.h_out + c) = v;
-			} else {
+		if (pair) {
+			// every row's load before the first store (h_in and h_out are
+			// different layers), so the band's reads are in flight together
+			double2 v[TILE];
+#pragma unroll
+			for (int r = 0; r < TILE; ++r)
+				if (r < nr) v[r] = *reinterpret_cast<const double2*>(p.h_in + (size_t)(i0 + r) * p.cols + j);
+#pragma unroll
+			for (int r = 0; r < TILE; ++r)
+				if (r < nr) {
+					v[r].x += 0.0;   // h + (0.0 - 0.0) x 4: every flow around the cell is zero
+					v[r].y += 0.0;
+					*reinterpret_cast<double2*>(p.h_out + (size_t)(i0 + r) * p.cols + j) = v[r];
+				}
+		} else {
+			for (int r = 0; r < nr; ++r) {
+				const size_t c = (size_t)(i0 + r) * p.cols + j;
 				if (c0) p.h_out[c] = p.h_in[c] + 0.0;
 				if (c1) p.h_out[c + 1] = p.h_in[c + 1] + 0.0;
 			}
