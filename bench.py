@@ -225,7 +225,9 @@ def bench_active(n=1024, steps=4000, seed=7, cpu_steps=400):
 def bench_wet(n=4096, steps=100, seed=2024):
     """SciddicaT 4096^2 full grid with debris on every cell (h ~ U(0.5, 5) m,
     the headline's DEM): the minimisation runs everywhere, the tile mode
-    declines and the engine takes its strip sweeps (two steps per launch)."""
+    declines and the engine takes its strip sweeps (one or two steps per
+    launch, whichever its first-use timing of the grid shape found faster;
+    one for this wet grid)."""
     import native
     from opencal_amd import synth
     from opencal_amd.models import SciddicaT
@@ -253,7 +255,7 @@ def bench_wet(n=4096, steps=100, seed=2024):
             "achieved_gbs_per_launch": MIN_BYTES_PER_CELL_SCIDDICAT * n * n / (ktime_ms / max(launches, 1) * 1e-3) / 1e9,
             "wet_fraction_at_end": wet, "bit_exact_vs_oracle": bool(np.array_equal(gh, oh)),
             "note": "wall time with per-launch HIP events; achieved = 16 B per cell over the average launch "
-                    "(each launch reads z, h and writes h for two steps: 24 B per cell)"}
+                    "(a strip sweep reads z and h and writes h: 24 B per cell and launch)"}
 
 
 def bench_life(n=512, steps=1000, seed=1):
