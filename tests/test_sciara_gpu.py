@@ -274,3 +274,48 @@ def test_sciara_two_launch_vent_on_border():
         assert np.array_equal(a[k], b[k]), k
         assert np.array_equal(a[k], ora[k]), k
     assert a["active"] == b["active"] == ora["active"]
+
+
+def test_sciara_two_launch_host_edits_mid_run():
+    """Host edits of the set between steps (a far dry cell added, a lava cell
+    removed) end the two-launch mode: the engine finishes the fused
+    transitions, the next step applies the edits through the four launches
+    (the vent process still sees the old list, then the rebuild), and the
+    two-launch mode resumes.  Bit-exact against the four-launch path with
+    the same edits."""
+    import ctypes as C
+    import os
+    from opencal_amd import lib
+    from opencal_amd.models import Sciara
+    inp = _inputs(96, 128, 11, [[30, 50], [44, 90]], steps_per_rate=60, n_rates=4)
+    L = lib()
+    L.calAddActiveCell2D.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.calRemoveActiveCell2D.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    res = {}
+    for tag, env in (("two", TWO), ("four", FOUR)):
+        old = os.environ.get("CALHIP_SCIARA_TWO_LAUNCH")
+        os.environ.update(env)
+        m = Sciara(inp, 300, True)
+        try:
+            m.init()
+            ran = m.steps(50)
+            L.calAddActiveCell2D(m.model, 80, 10)       # far from the lava, dry
+            ran += m.steps(40)
+            L.calRemoveActiveCell2D(m.model, 30, 50)    # a vent's cell, wet
+            ran += m.steps(60)
+            out = m.read()
+            out["ran"] = ran
+            out["launches"] = m.stats()["launches"]
+            res[tag] = out
+        finally:
+            m.close()
+            if old is None:
+                os.environ.pop("CALHIP_SCIARA_TWO_LAUNCH", None)
+            else:
+                os.environ["CALHIP_SCIARA_TWO_LAUNCH"] = old
+    a, b = res["two"], res["four"]
+    print(f"launches two-launch {a['launches']}, four-launch {b['launches']}, active {a['active']}", flush=True)
+    for k in ("z", "h", "T", "Msl"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["active"] == b["active"] and a["elapsed"] == b["elapsed"] and a["ran"] == b["ran"] == 150
+    assert a["launches"] < b["launches"]
