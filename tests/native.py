@@ -164,6 +164,46 @@ def ref_sciara(inp, steps, active=True):
     return _sciara_call(lib.ref_sciara_run, inp, steps, active, ())
 
 
+def oracle_s3hex(z, d, s, steps, order=0):
+    """The S3 debris flow of cal_sciddicaS3hex-unsafe-explicit-glut on the
+    hexagonal torus; order 0 visits the cells serially, 1 runs the flows in
+    the device's colour phases.  Returns a dict of the current layers h, p,
+    z, d (float64) and s (int32) after `steps` steps."""
+    z = np.ascontiguousarray(z, np.float64)
+    d = np.ascontiguousarray(d, np.float64)
+    s = np.ascontiguousarray(s, np.int32)
+    rows, cols = z.shape
+    out = {k: np.empty_like(z) for k in ("h", "p", "z", "d")}
+    out["s"] = np.empty_like(s)
+    rc = oracle().orc_s3hex_run(rows, cols, dptr(z), dptr(d), iptr(s), int(steps), int(order), dptr(out["h"]),
+                                dptr(out["p"]), dptr(out["z"]), dptr(out["d"]), iptr(out["s"]))
+    assert rc == 0
+    return out
+
+
+def s3hex_inputs(rows, cols, seed):
+    """Synthetic inputs of the S3 example's shape: altitude (the seeded DEM),
+    regolith depth (-9999 outside an erodible band, as the example's
+    regolith.txt marks no-data, -1 on rock, 2 m in the band with 0.5 m
+    patches) and the source countdown (0, with 16 cells of 6 m regolith at
+    -1 .. -6: each starts a landslide when its countdown reaches -1)."""
+    from opencal_amd import synth
+    z = synth.dem(rows, cols, seed, base=1000.0)
+    u = synth.uniform(seed + 11, rows * cols).reshape(rows, cols)
+    d = np.full((rows, cols), -9999.0)
+    r0, r1 = rows // 8, rows // 2
+    c0, c1 = cols // 4, 3 * cols // 4
+    d[r0:r1, c0:c1] = np.where(u[r0:r1, c0:c1] < 0.15, -1.0, np.where(u[r0:r1, c0:c1] < 0.3, 0.5, 2.0))
+    s = np.zeros((rows, cols), np.int32)
+    v = synth.uniform(seed + 12, 16)
+    for k in range(16):
+        i = r0 + 2 + int(v[k] * (r1 - r0 - 4))
+        j = c0 + 2 + int(((k * 0.618034) % 1.0) * (c1 - c0 - 4))
+        s[i, j] = -1 - (k % 6)
+        d[i, j] = 6.0
+    return z, d, s
+
+
 def oracle_sciddicat_unsafe(z, h, p_r, p_eps, steps, order=0):
     """The unsafe SciddicaT (next-layer read-modify-writes to the
     neighbours, active cells) of cal_sciddicaT-unsafe-glut; order 0 visits

@@ -248,3 +248,44 @@ def test_reference_unsafe_program_on_synthetic_data(tmp_path):
     assert np.array_equal(got, phased)
     assert np.abs(got - serial).max() <= 1e-12 * h.max()
     assert abs(got.sum() - h.sum()) <= 1e-12 * h.sum()
+
+
+@pytest.mark.parametrize("variant,seed,steps", [("", 3, 400), ("", 8, 150), ("_implicit", 3, 400)])
+def test_reference_s3hex_program_on_the_generic_device_program(tmp_path, variant, seed, steps):
+    """The reference example cal_sciddicaS3hex-unsafe-explicit-glut (its
+    model file unchanged: hexagonal torus, its own global transition of four
+    calApplyElementaryProcess2D calls each followed by calUpdateSubstate2Dr,
+    a flows process that writes its outflows into the neighbours' next
+    layers of h and p, calSetCurrent2D on its single-layer substates), its
+    four processes compiled for the GPU at run time; the flows run in colour
+    phases.  The example ships no DEM, so the inputs are synthetic rasters of
+    its 767 x 925 shape.  Every substate (h, p, z, d and the int countdown s)
+    equals the oracle's colour-phase visit bit for bit; the oracle's serial
+    visit is the reference program's own result on the reference engine
+    (tests/test_oracle_s3hex.py).  The implicit twin
+    cal_sciddicaS3hex-unsafe-glut (variant "_implicit": the same processes
+    under calGlobalTransitionFunction2D with an update after each) gives the
+    same bits."""
+    import numpy as np
+    BIN_S3 = os.path.join(native.REF_DIR, "dropin_s3hex" + variant)
+    if not os.path.exists(BIN_S3):
+        pytest.skip(f"dropin_s3hex{variant} not built (needs the reference tree)")
+    print(f"case dropin s3hex{variant} 767x925 seed {seed}, {steps} steps", flush=True)
+    z, d, s = native.s3hex_inputs(767, 925, seed)
+    data = tmp_path / "data"
+    data.mkdir()
+    np.savetxt(data / "dem.txt", z, fmt="%.17g")
+    np.savetxt(data / "regolith.txt", d, fmt="%.17g")
+    np.savetxt(data / "source.txt", s, fmt="%d")
+    r = subprocess.run([BIN_S3, str(tmp_path / "got"), str(steps)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"steps {steps}" in r.stdout, r.stdout + r.stderr
+    assert (data / "width_final.txt").exists()
+    got = {k: np.fromfile(tmp_path / f"got_{k}.bin", np.float64).reshape(767, 925) for k in ("h", "p", "z", "d")}
+    got["s"] = np.fromfile(tmp_path / "got_s.bin", np.int32).reshape(767, 925)
+    want = native.oracle_s3hex(z, d, s, steps, 1)
+    for k in ("h", "p", "z", "d", "s"):
+        assert np.array_equal(got[k], want[k]), f"{k}: max diff {np.abs(got[k] - want[k]).max()}"
+    # the sixteen slides started and spread (a film of debris on thousands of cells)
+    assert (got["z"] < z).sum() >= 16 and (got["h"] > 0).sum() > 1000
