@@ -289,3 +289,25 @@ def test_reference_s3hex_program_on_the_generic_device_program(tmp_path, variant
         assert np.array_equal(got[k], want[k]), f"{k}: max diff {np.abs(got[k] - want[k]).max()}"
     # the sixteen slides started and spread (a film of debris on thousands of cells)
     assert (got["z"] < z).sum() >= 16 and (got["h"] > 0).sum() > 1000
+
+
+BIN_MOD2 = os.path.join(native.REF_DIR, "dropin_mod2CA3D")
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_MOD2), reason="dropin_mod2CA3D not built (needs the reference tree)")
+def test_reference_mod2_3d_program_on_the_generic_device_program(tmp_path):
+    """The reference 3D example cal_mod2CA3D-glut (mod2CA3D.c unchanged:
+    65^3 torus, Moore 3D, a byte substate, the parity of each 27-cell sum,
+    one seed cell, 1000 steps), its process compiled for the GPU at run time:
+    the final state is the closed form's (the outer product of the 1D
+    trinomial parities), which the reference program on the reference engine
+    also reaches (tests/test_engine3d.py)."""
+    import numpy as np
+    from test_engine3d import mod2_closed_form
+    print("case dropin mod2CA3D 65^3 1000 steps", flush=True)
+    out = tmp_path / "mod2.bin"
+    r = subprocess.run([BIN_MOD2, str(out)], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "steps 1000 live 79507" in r.stdout, r.stdout + r.stderr
+    got = np.fromfile(out, np.uint8).reshape(65, 65, 65)
+    assert np.array_equal(got, mod2_closed_form((65, 65, 65), (12, 12, 12), 1000))

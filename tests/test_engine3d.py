@@ -16,6 +16,7 @@ before the model is on the GPU (host arrays staged) and after."""
 import ctypes as C
 import math
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -639,3 +640,20 @@ def oracle_heat_from(T0, hot0, count0, kappa, thr):
     hot = (T > thr).astype(np.uint8)
     count = np.where(T > thr, count0 + 1, count0).astype(np.int32)
     return T, hot, count, None
+
+
+MOD2_REF = os.path.join(native.REF_DIR, "calomp_mod2CA3D")
+
+
+@pytest.mark.skipif(not os.path.exists(MOD2_REF), reason="calomp_mod2CA3D not built (needs the reference tree)")
+def test_reference_mod2_program_matches_closed_form(tmp_path):
+    """The reference example cal_mod2CA3D-glut (mod2CA3D.c unchanged, 65^3
+    torus, seed at (12, 12, 12), 1000 steps) on the reference 3D engine ends
+    in the closed form's state, cell for cell."""
+    out = tmp_path / "mod2.bin"
+    r = subprocess.run([MOD2_REF, str(out)], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8")))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "steps 1000" in r.stdout
+    got = np.fromfile(out, np.uint8).reshape(65, 65, 65)
+    assert np.array_equal(got, mod2_closed_form((65, 65, 65), (12, 12, 12), 1000))
