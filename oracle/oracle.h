@@ -32,6 +32,10 @@ int orc_sciddicat_run(int rows, int cols, const double* z, const double* h,
  * init (which lowers z where h > 0); out_active: the final list length. */
 int orc_sciddicat_unsafe_run(int rows, int cols, const double* z0, const double* h0, double p_r, double p_eps,
                              int steps, int order, double* out_h, int* out_active);
+/* The same; track 0: cal_sciddicaT-unsafe (ACTIVE_CELLS not defined, the
+ * list stays empty and every sweep covers the grid). */
+int orc_sciddicat_unsafe_run_track(int rows, int cols, const double* z0, const double* h0, double p_r,
+                                   double p_eps, int steps, int order, int track, double* out_h, int* out_active);
 
 /* Conway Life, Moore, toroidal, uint8 0/1 cells. */
 int orc_life_run(int rows, int cols, const unsigned char* alive0, int steps,

@@ -25,8 +25,15 @@
  * at least three rows or three columns apart on the torus, so their
  * read-modify-writes never touch the same cell.  This is the order of the
  * device's colour-phased unsafe processes.
+ * With track 0 the model is OpenCAL/examples/cal_sciddicaT-unsafe
+ * (source/sciddicaT.c:36-130: the same processes, ACTIVE_CELLS not defined,
+ * the model still declared with CAL_OPT_ACTIVE_CELLS, :139): no cell is ever
+ * added, the list stays empty, so every sweep and every update covers the
+ * whole grid (cal2D.cpp:356-361 and :651-663).
  * Pinned by: the shipped cal_sciddicaT-unsafe-glut/data/width_final.txt
- * (610 x 496, 4000 steps, "%f"), mass conservation on the torus. */
+ * (610 x 496, 4000 steps, "%f"), mass conservation on the torus, and the
+ * reference programs of both examples built on the reference OpenCAL-OMP
+ * engine (one thread: the serial order). */
 #include "oracle.h"
 
 #include <stdlib.h>
@@ -39,6 +46,7 @@ typedef struct {
 	unsigned char* flag;
 	int* cells;
 	int n_cells;
+	int track; /* ACTIVE_CELLS defined: the processes add and remove cells */
 	double r, eps;
 } su_t;
 
@@ -80,7 +88,7 @@ static void su_transition(su_t* s, int i, int j)
 			f = (avg - u[n]) * s->r;
 			s->hn[c] = s->hn[c] - f;
 			s->hn[nb[n]] = s->hn[nb[n]] + f;
-			s->flag[nb[n]] = 1;
+			if (s->track) s->flag[nb[n]] = 1;
 		}
 }
 
@@ -129,6 +137,17 @@ static void su_sweep(su_t* s, int order)
 	{
 		int ci, cj;
 		const int nci = 3 + s->rows % 3, ncj = 3 + s->cols % 3;
+		const int fi = s->rows / 3 * 3, fj = s->cols / 3 * 3;
+		if (all) {
+			/* the same phases, each class's rows and columns enumerated */
+			int i, j;
+			for (ci = 0; ci < nci; ci++)
+				for (cj = 0; cj < ncj; cj++)
+					for (i = ci < 3 ? ci : fi + ci - 3; i < (ci < 3 ? fi : fi + ci - 2); i += ci < 3 ? 3 : 1)
+						for (j = cj < 3 ? cj : fj + cj - 3; j < (cj < 3 ? fj : fj + cj - 2); j += cj < 3 ? 3 : 1)
+							su_transition(s, i, j);
+			return;
+		}
 		for (ci = 0; ci < nci; ci++)
 			for (cj = 0; cj < ncj; cj++)
 				for (k = 0; k < cnt; k++) {
@@ -139,8 +158,8 @@ static void su_sweep(su_t* s, int order)
 	}
 }
 
-int orc_sciddicat_unsafe_run(int rows, int cols, const double* z0, const double* h0, double p_r, double p_eps,
-                             int steps, int order, double* out_h, int* out_active)
+int orc_sciddicat_unsafe_run_track(int rows, int cols, const double* z0, const double* h0, double p_r,
+                                   double p_eps, int steps, int order, int track, double* out_h, int* out_active)
 {
 	su_t s;
 	const int n = rows * cols;
@@ -155,6 +174,7 @@ int orc_sciddicat_unsafe_run(int rows, int cols, const double* z0, const double*
 	s.cells = (int*)malloc((size_t)n * sizeof(int));
 	s.r = p_r;
 	s.eps = p_eps;
+	s.track = track != 0;
 	if (!z || !s.hc || !s.hn || !s.flag || !s.cells) return -1;
 	memcpy(z, z0, (size_t)n * sizeof(double));
 	memcpy(s.hc, h0, (size_t)n * sizeof(double));
@@ -163,15 +183,16 @@ int orc_sciddicat_unsafe_run(int rows, int cols, const double* z0, const double*
 	for (c = 0; c < n; c++)
 		if (s.hc[c] > 0.0) {
 			z[c] = z[c] - s.hc[c];
-			s.flag[c] = 1;
+			if (s.track) s.flag[c] = 1;
 		}
 	s.z = z;
 	su_update(&s);
 	for (t = 0; t < steps; t++) {
 		su_sweep(&s, order);
 		su_update(&s);
-		/* sciddicaT_remove_inactive_cells (:88-94) over the list */
-		{
+		/* sciddicaT_remove_inactive_cells (:88-94) over the list; an empty
+		 * process without ACTIVE_CELLS */
+		if (s.track) {
 			const int all = s.n_cells == 0, cnt = all ? n : s.n_cells;
 			for (k = 0; k < cnt; k++) {
 				const int cc = all ? k : s.cells[k];
@@ -188,4 +209,10 @@ int orc_sciddicat_unsafe_run(int rows, int cols, const double* z0, const double*
 	free(s.flag);
 	free(s.cells);
 	return 0;
+}
+
+int orc_sciddicat_unsafe_run(int rows, int cols, const double* z0, const double* h0, double p_r, double p_eps,
+                             int steps, int order, double* out_h, int* out_active)
+{
+	return orc_sciddicat_unsafe_run_track(rows, cols, z0, h0, p_r, p_eps, steps, order, 1, out_h, out_active);
 }

@@ -204,17 +204,19 @@ def s3hex_inputs(rows, cols, seed):
     return z, d, s
 
 
-def oracle_sciddicat_unsafe(z, h, p_r, p_eps, steps, order=0):
+def oracle_sciddicat_unsafe(z, h, p_r, p_eps, steps, order=0, track=True):
     """The unsafe SciddicaT (next-layer read-modify-writes to the
     neighbours, active cells) of cal_sciddicaT-unsafe-glut; order 0 visits
-    the cells serially, 1 in the device's colour phases.  Returns (h, final
-    active-list length)."""
+    the cells serially, 1 in the device's colour phases.  track=False is
+    cal_sciddicaT-unsafe (no cell ever added: full-grid sweeps).  Returns
+    (h, final active-list length)."""
     z = np.ascontiguousarray(z, np.float64)
     h = np.ascontiguousarray(h, np.float64)
     rows, cols = z.shape
     oh = np.empty_like(h)
     na = C.c_int(0)
-    rc = oracle().orc_sciddicat_unsafe_run(rows, cols, dptr(z), dptr(h), C.c_double(p_r),
-                                           C.c_double(p_eps), steps, order, dptr(oh), C.byref(na))
+    rc = oracle().orc_sciddicat_unsafe_run_track(rows, cols, dptr(z), dptr(h), C.c_double(p_r),
+                                                 C.c_double(p_eps), steps, order, int(track), dptr(oh),
+                                                 C.byref(na))
     assert rc == 0
     return oh, na.value

@@ -250,6 +250,46 @@ def test_reference_unsafe_program_on_synthetic_data(tmp_path):
     assert abs(got.sum() - h.sum()) <= 1e-12 * h.sum()
 
 
+BIN_UP = os.path.join(native.REF_DIR, "dropin_sciddicaT_unsafe_plain")
+
+
+@pytest.mark.skipif(not os.path.exists(BIN_UP),
+                    reason="dropin_sciddicaT_unsafe_plain not built (needs the reference tree)")
+@pytest.mark.parametrize("data", ["example", "synthetic"])
+def test_reference_full_grid_unsafe_program(tmp_path, data):
+    """The reference example cal_sciddicaT-unsafe, its file unchanged
+    (sciddicaT.c:86-87 writes each outflow into the neighbour's next layer
+    with calSetX2Dr / calGetNextX2Dr; ACTIVE_CELLS is not defined, so the
+    active set declared with CAL_OPT_ACTIVE_CELLS stays empty and every
+    sweep covers the grid), its own main and calRun2D with only the bind
+    calls added; the transition runs in colour phases.  Its "%.9f" file is
+    the oracle's of the full-grid model in phase order: on the example's
+    data, the file the reference engine writes for it; on a synthetic
+    slope, within 1e-12 of the serial order."""
+    import numpy as np
+    from opencal_amd import synth
+    print(f"case dropin sciddicaT unsafe full grid 610x496 4000 steps {data}", flush=True)
+    if data == "example":
+        _unsafe_data(tmp_path)
+        z = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_dem.txt.gz"))
+        h = synth.read_matrix_gz(os.path.join(GOLDEN, "sciddicaT_610x496_source.txt.gz"))
+    else:
+        z, h, _ = synth.sciddicat_inputs(610, 496, seed=7, patch=12)
+        _unsafe_data(tmp_path, z, h)
+    r = subprocess.run([BIN_UP], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Simulation terminated." in r.stdout, r.stdout + r.stderr
+    got = (tmp_path / "data" / "width_final.txt").read_text()
+    if data == "example":
+        assert got == gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
+        return
+    phased, n = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 4000, 1, track=False)
+    assert n == 0
+    assert got == "".join("".join("%.9f  " % v for v in row) + "\n" for row in phased)
+    vals = np.array([[float(x) for x in line.split()] for line in got.splitlines()])
+    assert abs(vals.sum() - h.sum()) <= 1e-6 * h.sum()
+
+
 @pytest.mark.parametrize("variant,seed,steps", [("", 3, 400), ("", 8, 150), ("_implicit", 3, 400)])
 def test_reference_s3hex_program_on_the_generic_device_program(tmp_path, variant, seed, steps):
     """The reference example cal_sciddicaS3hex-unsafe-explicit-glut (its

@@ -13,6 +13,16 @@ with active cells), pinned by:
 - mass conservation on the torus (every outflow leaves one cell and enters
   another).
 
+With track=False the oracle is OpenCAL/examples/cal_sciddicaT-unsafe (the
+same processes with ACTIVE_CELLS not defined: no cell is ever added, every
+sweep covers the grid), pinned by that program built on the reference
+OpenCAL-OMP engine (oracle/_ref/calomp_sciddicaT_unsafe_plain, one thread),
+which writes the same "%.9f" file as the glut program.  A cell with
+h <= eps sends nothing (once the cell itself is eliminated the average is
+the remaining neighbours' mean plus m / k < their minimum), so the full-grid
+sweep writes what the active-cell sweep writes, in the same row-major order:
+both modes agree bit for bit.
+
 order 1 restates the device's colour-phased visit; it differs from the
 serial visit only in the order in which a cell's next value receives its
 neighbours' outflows, i.e. by rounding."""
@@ -29,6 +39,7 @@ from opencal_amd import synth
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 REF_BIN = os.path.join(native.REF_DIR, "calomp_sciddicaT_unsafe")
+REF_PLAIN = os.path.join(native.REF_DIR, "calomp_sciddicaT_unsafe_plain")
 
 
 def _example_inputs():
@@ -81,6 +92,39 @@ def test_phase_order_differs_from_serial_only_by_rounding(shape, seed):
     for out in (a, b):
         assert abs(out.sum() - h.sum()) <= 1e-12 * h.sum()
         assert out.min() > -1e-12
+
+
+@pytest.mark.parametrize("shape,seed", [((64, 64), 1), ((67, 71), 2), ((31, 29), 4), ((5, 7), 6), ((4, 5), 3),
+                                        ((2, 8), 1)])
+def test_full_grid_mode_equals_the_active_cell_mode(shape, seed):
+    """track=False (every sweep over the grid) against track=True, both
+    visit orders, on wet and dry cells alike."""
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(0, 50, shape).cumsum(0) * 0.1
+    h = np.where(rng.random(shape) < 0.4, rng.uniform(0, 5, shape), 0.0)
+    for order in (0, 1):
+        a, _ = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 300, order)
+        b, n = native.oracle_sciddicat_unsafe(z, h, 0.5, 0.001, 300, order, track=False)
+        assert np.array_equal(a, b) and n == 0, order
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PLAIN),
+                    reason="calomp_sciddicaT_unsafe_plain not built (needs the reference tree)")
+def test_reference_full_grid_program_writes_the_oracle_output(tmp_path):
+    """cal_sciddicaT-unsafe's own program (its main: 610 x 496, 4000 steps,
+    calRun2D) on the reference OpenCAL-OMP engine, one thread (about 90 s):
+    its "%.9f" file is the oracle's, which is the glut program's."""
+    z, h = _example_inputs()
+    data = tmp_path / "data"
+    data.mkdir()
+    np.savetxt(data / "dem.txt", z, fmt="%.17g")
+    np.savetxt(data / "source.txt", h, fmt="%.17g")
+    r = subprocess.run([REF_PLAIN], cwd=tmp_path, capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = (data / "width_final.txt").read_text()
+    assert got == _format(_example_run(0)[0], "%.9f")
+    assert got == gzip.open(os.path.join(GOLDEN, "sciddicaT_610x496_width_final.txt.gz"), "rt").read()
 
 
 def _run_reference(tmp_path, z, h):
