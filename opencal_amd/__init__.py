@@ -101,6 +101,7 @@ def _declare_examples(E):
     E.sciddicat_read.argtypes = [vp, dp, dp, ip]
     E.sciddicat_set_explicit.argtypes = [vp, C.c_int]
     E.sciddicat_read_flow.argtypes = [vp, C.c_int, dp]
+    E.sciddicat_set_h.argtypes = [vp, C.c_int, C.c_int, C.c_double]
     E.sciddicat_model.restype = vp
     E.sciddicat_model.argtypes = [vp]
     E.sciddicat_destroy.argtypes = [vp]

@@ -499,6 +499,7 @@ struct SciddicaTProgram : Program {
 		unsigned* tag[2] = {nullptr, nullptr};
 		int* list[2] = {nullptr, nullptr};
 		int* cnt = nullptr;        // [3] list lengths by launch index mod 3
+		unsigned* same = nullptr;  // per tile: first launch of its run of copies (0: none)
 		char* block = nullptr;
 		int* h_cnt = nullptr;      // pinned copy of a list length (read without a stream sync)
 		hipEvent_t ev = nullptr;
@@ -796,7 +797,7 @@ struct SciddicaTProgram : Program {
 			if (CALHIP_CHECK(hipDeviceGetAttribute(&pl.cus, hipDeviceAttributeMultiprocessorCount, m->device)))
 				return -1;
 			const size_t nt = (size_t)pl.n_tiles;
-			const size_t bytes = 2 * nt * sizeof(unsigned) + 2 * nt * sizeof(int) + 4 * sizeof(int);
+			const size_t bytes = 3 * nt * sizeof(unsigned) + 2 * nt * sizeof(int) + 4 * sizeof(int);
 			if (CALHIP_CHECK(hipMalloc(&pl.block, bytes))) return -1;
 			if (CALHIP_CHECK(hipMemsetAsync(pl.block, 0, bytes, m->stream))) return -1;
 			pl.tag[0] = reinterpret_cast<unsigned*>(pl.block);
@@ -804,12 +805,14 @@ struct SciddicaTProgram : Program {
 			pl.list[0] = reinterpret_cast<int*>(pl.tag[1] + nt);
 			pl.list[1] = pl.list[0] + nt;
 			pl.cnt = pl.list[1] + nt;
+			pl.same = reinterpret_cast<unsigned*>(pl.cnt + 4);
 			if (CALHIP_CHECK(hipHostMalloc((void**)&pl.h_cnt, sizeof(int), hipHostMallocDefault))) return -1;
 			if (CALHIP_CHECK(hipEventCreateWithFlags(&pl.ev, hipEventDisableTiming))) return -1;
 			pl_e = 0;
 		}
 		if (pl_e >= 0xfffffff0u) {
-			// the tags would wrap: every tile back to "never listed"
+			// the tags would wrap: every tile back to "never listed" (and, by
+			// the setup below, to "no copy yet")
 			if (CALHIP_CHECK(hipMemsetAsync(pl.tag[0], 0, 2 * (size_t)pl.n_tiles * sizeof(unsigned), m->stream)))
 				return -1;
 			pl_e = 0;
@@ -852,6 +855,7 @@ struct SciddicaTProgram : Program {
 		p.cnt_in = pl.cnt + pl_L % 3;
 		p.cnt_out = pl.cnt + (pl_L + 1) % 3;
 		p.cnt_clear = pl.cnt + (pl_L + 2) % 3;
+		p.same = pl.same;
 		p.e = pl_e;
 		p.rows = m->pub.rows;
 		p.cols = m->pub.columns;
@@ -886,8 +890,9 @@ struct SciddicaTProgram : Program {
 			const int Kn = pl_pick(n - done > K ? std::min(n - done - K, pl.km) : pl.km);
 			actf::PlainLaunch p = pl_args(m);
 			p.km = Kn;
-			// compute blocks: as many as fit beside each other (LDS: one per CU from K = 8)
-			p.grid_comp = std::max(1, std::min(pl.n_tiles, pl.cus * (K >= 8 ? 1 : 2)));
+			// compute blocks: as many as fit beside each other (two per CU: up
+			// to 1024 threads and 40 KiB of LDS each)
+			p.grid_comp = std::max(1, std::min(pl.n_tiles, pl.cus * 2));
 			hipEvent_t ev;
 			timer_begin(m, &ev);
 			if (actf::launch_plain(m->stream, K, p)) return -1;

@@ -323,13 +323,17 @@ struct PlainShape {
 };
 
 // Lists every tile within p.km of a wet cell (h > eps or NaN) of the grid in
-// list_out with tag e; cnt_out is zero on entry.
+// list_out with tag e; cnt_out is zero on entry.  Every tile starts with no
+// copy behind it (same = 0).
 __global__ __launch_bounds__(TILE * TILE) void plain_setup(PlainLaunch p)
 {
 	__shared__ unsigned sh_mark;
 	const int tid = threadIdx.x, la = tid / TILE, lb = tid % TILE;
 	for (int t = blockIdx.x; t < p.n_tiles; t += gridDim.x) {
-		if (tid == 0) sh_mark = 0u;
+		if (tid == 0) {
+			sh_mark = 0u;
+			p.same[t] = 0u;
+		}
 		__syncthreads();
 		const int ti = t % p.ty, tj = t / p.ty;
 		const int tre = min(TILE, p.rows - ti * TILE), tce = min(TILE, p.cols - tj * TILE);
@@ -347,6 +351,16 @@ __global__ __launch_bounds__(TILE * TILE) void plain_setup(PlainLaunch p)
 // Blocks [0, grid_comp): K steps of the listed tiles.  The others: the
 // copy h + 0.0 of every unlisted tile, each block a band of 16 rows (one
 // tile row) and 2 * NT columns, two cells per thread and row.
+//
+// An unlisted tile stays as it is (every flow around its cells is zero): h'
+// = h + 0.0, which only turns -0.0 into +0.0.  Once it has been copied by
+// two consecutive launches, both layers hold h + 0.0 of the same values, and
+// every later launch would write what the layer it writes already holds: the
+// copy of such a tile is left out until the tile is listed again.  same[t]
+// holds the e of the first launch of the tile's run of copies (0: listed
+// since, or no copy yet); a launch copies the tile unless 0 < same[t] <=
+// e - 2.  The thread that finds 0 stores e: threads of the same launch that
+// read it afterwards see e and copy as well.
 template <int K>
 __global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
 {
@@ -367,9 +381,15 @@ __global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
 		if (ti >= p.ty) return;
 		const int j = bc * 2 * NT + 2 * tid;
 		if (j >= p.cols) return;
-		const int tj0 = j / TILE, tj1 = (j + 1) / TILE;
+		const int tj0 = j / TILE, tj1 = (j + 1) / TILE;   // j even: one tile
 		const bool c0 = p.tag_in[ti + tj0 * p.ty] != p.e;
 		const bool c1 = j + 1 < p.cols && p.tag_in[ti + tj1 * p.ty] != p.e;
+		if (c0) {
+			unsigned* sm = p.same + ti + tj0 * p.ty;
+			const unsigned first = *sm;
+			if (first != 0u && first + 2u <= p.e) return;
+			if (first == 0u) *sm = p.e;
+		}
 		const int i0 = ti * TILE, nr = min(TILE, p.rows - i0);
 		const bool pair = c0 && c1 && (p.cols % 2 == 0);   // j even: 16-byte aligned pair
 		if (pair) {
@@ -472,6 +492,7 @@ __global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
 			}
 		}
 		if (mk) atomicOr(&sh_mark, mk);
+		if (tid == 0) p.same[t] = 0u;   // the layers differ from here
 		__syncthreads();
 		const unsigned marks = sh_mark;
 		__syncthreads();   // read before the next tile clears it
@@ -535,12 +556,7 @@ int plain_threads(int K)
 	case 1: return PlainShape<1>::NT;
 	case 2: return PlainShape<2>::NT;
 	case 3: return PlainShape<3>::NT;
-	case 4: return PlainShape<4>::NT;
-	case 5: return PlainShape<5>::NT;
-	case 6: return PlainShape<6>::NT;
-	case 7: return PlainShape<7>::NT;
-	case 8: return PlainShape<8>::NT;
-	default: return PlainShape<9>::NT;
+	default: return PlainShape<4>::NT;
 	}
 }
 
@@ -566,7 +582,7 @@ static void plain_go(hipStream_t st, int grid, const PlainLaunch& p)
 int launch_plain(hipStream_t st, int K, const PlainLaunch& p)
 {
 	if (K < 1 || K > PLAIN_KMAX) {
-		set_error(13, "SciddicaT full-grid launch: steps per launch must be 1 to 9");
+		set_error(13, "SciddicaT full-grid launch: steps per launch must be 1 to 4");
 		return -1;
 	}
 	const int grid = p.grid_comp + plain_copy_blocks(K, p.rows, p.cols);
@@ -574,12 +590,7 @@ int launch_plain(hipStream_t st, int K, const PlainLaunch& p)
 	case 1: plain_go<1>(st, grid, p); break;
 	case 2: plain_go<2>(st, grid, p); break;
 	case 3: plain_go<3>(st, grid, p); break;
-	case 4: plain_go<4>(st, grid, p); break;
-	case 5: plain_go<5>(st, grid, p); break;
-	case 6: plain_go<6>(st, grid, p); break;
-	case 7: plain_go<7>(st, grid, p); break;
-	case 8: plain_go<8>(st, grid, p); break;
-	default: plain_go<9>(st, grid, p); break;
+	default: plain_go<4>(st, grid, p); break;
 	}
 	return CALHIP_CHECK(hipGetLastError());
 }

@@ -117,6 +117,7 @@ struct PlainLaunch {
 	const int* cnt_in;
 	int* cnt_out;              // zero on entry
 	int* cnt_clear;            // zeroed by the launch (the next launch's cnt_out)
+	unsigned* same;            // per tile: e of the first of its consecutive copies (0: none)
 	unsigned e;
 	int rows, cols, ty, tx, n_tiles;
 	int km;                    // marking reach: the next launch's steps
@@ -124,7 +125,7 @@ struct PlainLaunch {
 	double r, eps;
 };
 
-constexpr int PLAIN_KMAX = 9;   // most steps per full-grid launch (1 .. 9)
+constexpr int PLAIN_KMAX = 4;   // most steps per full-grid launch (1 .. 4)
 
 // threads per block of the K-step kernel and the column width of one copy block
 int plain_threads(int K);

@@ -292,6 +292,14 @@ int sciddicat_read_flow(void* handle, int k, double* out)
 
 struct CALModel2D* sciddicat_model(void* handle) { return ((sciddicat_t*)handle)->model; }
 
+/* A host edit between steps: h = v at (i, j) in both layers (calInit2Dr). */
+int sciddicat_set_h(void* handle, int i, int j, double v)
+{
+	sciddicat_t* s = (sciddicat_t*)handle;
+	calInit2Dr(s->model, s->Q.h, i, j, v);
+	return calhipGetLastError() ? -1 : 0;
+}
+
 void sciddicat_destroy(void* handle)
 {
 	sciddicat_t* s = (sciddicat_t*)handle;

@@ -83,6 +83,10 @@ class SciddicaT:
     def stats(self):
         return stats(self.model)
 
+    def set_h(self, i, j, v):
+        """A host edit between steps: h = v at (i, j), both layers."""
+        check(self.E.sciddicat_set_h(self.handle, int(i), int(j), C.c_double(v)), "sciddicat_set_h")
+
     def timing(self, enable):
         """1: events around each sweep launch; 2: one event pair around the
         whole region (launches back to back); 0: off."""

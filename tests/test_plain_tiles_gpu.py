@@ -1,8 +1,9 @@
-"""SciddicaT on the full grid (CAL_NO_OPT) in launches of up to 9 steps: the
-tiles near wet cells advance in LDS, every other tile is copied (h + 0.0),
-each launch reading and writing every cell once.  Checked bit for bit
-against the oracle (zero signs included) through calRun2D's loop, one
-calRunCAStep2D per step (grouped nine to a launch) and mixtures, on
+"""SciddicaT on the full grid (CAL_NO_OPT) in launches of up to 4 steps: the
+tiles near wet cells advance in LDS, every other tile is copied (h + 0.0)
+until two consecutive launches have copied it, after which both layers hold
+its values and it is left alone.  Checked bit for bit against the oracle
+(zero signs included) through calRun2D's loop, one calRunCAStep2D per step
+(grouped four to a launch), mixtures and host edits between runs, on
 tile-ragged grid shapes, with special values, and with the mode forced on
 (CALHIP_SCIDDICAT_TILES_ALWAYS) where the wet area would otherwise make the
 engine keep the strip sweeps."""
@@ -22,8 +23,10 @@ def same(a, b):
 
 
 def run_plan(z, h, p_r, p_eps, plan, env=None):
+    """plan: ("run", n) for n steps of calRun2D's loop, ("step", n) for n
+    calRunCAStep2D calls, ("set", [(i, j, v), ...]) for host edits of h."""
     from opencal_amd.models import SciddicaT
-    total = sum(n for _, n in plan)
+    total = sum(n for k, n in plan if k != "set")
     old = {k: os.environ.get(k) for k in (env or {})}
     os.environ.update(env or {})
     m = SciddicaT(z, h, p_r, p_eps, total + 8, active=False)
@@ -32,6 +35,9 @@ def run_plan(z, h, p_r, p_eps, plan, env=None):
         for kind, n in plan:
             if kind == "run":
                 m.run_steps(n)
+            elif kind == "set":
+                for i, j, v in n:
+                    m.set_h(i, j, v)
             else:
                 m.steps(n)
         launches = m.stats()["launches"]
@@ -51,7 +57,7 @@ ALWAYS = {"CALHIP_SCIDDICAT_TILES_ALWAYS": "1"}
 
 
 @pytest.mark.parametrize("rows,cols,steps,seed", [
-    (64, 64, 61, 1),      # whole tiles: 9 steps per launch
+    (64, 64, 61, 1),      # whole tiles: 4 steps per launch
     (37, 53, 45, 2),      # last tile row and column of 5 cells: up to 4 steps per launch
     (100, 70, 33, 3),
     (17, 17, 40, 4),      # two tile rows and columns, the second of one cell
@@ -74,13 +80,13 @@ def test_plain_tiles_vs_oracle(rows, cols, steps, seed):
 
 def test_plain_tiles_launches():
     """At 256 x 256 with a 12 x 12 source the mode turns on by itself and
-    advances nine steps per launch."""
+    advances four steps per launch."""
     z, h, _ = synth.sciddicat_inputs(256, 256, 10, patch=12)
     _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 81, False)
     gh, launches, _ = run_plan(z, h, 0.5, 0.001, [("run", 81)])
     print(f"launches {launches}", flush=True)
     assert same(gh, oh)
-    assert launches <= 1 + 9 + 3   # entry list, 80 steps in 9 launches (8 x 9 + 8), the first step's own
+    assert launches <= 1 + 20 + 3   # entry list, 80 steps in 20 launches, the first step's own
 
 
 @pytest.mark.parametrize("p_r,p_eps", [(0.3, 0.0), (0.9, 1e-2), (1.0, 1e-3)])
@@ -105,3 +111,27 @@ def test_plain_tiles_special_values():
         for plan in ([("run", steps)], [("step", steps)]):
             gh, _, _ = run_plan(z, h, 0.5, 0.001, plan, ALWAYS)
             assert same(gh, oh), f"steps={steps} plan={plan}"
+
+
+def test_plain_tiles_host_edits_between_launches():
+    """Dry tiles copied by two consecutive launches are left alone until they
+    are listed again (both layers already hold h + 0.0).  Host edits between
+    runs (-0.0, a new wet patch, a dry value) must reach both layers of
+    such tiles: the result equals the strip sweeps' (which the oracle pins)
+    bit for bit, zero signs included."""
+    z, h, (r0, c0, p) = synth.sciddicat_inputs(160, 224, 21, patch=10)
+    far = [(r, c) for r in range(120, 126) for c in range(150, 170)]
+    plan = [("run", 40),
+            ("set", [(r, c, -0.0) for r, c in far[:40]] + [(r, c, 3.5) for r, c in far[40:60]]),
+            ("run", 31), ("step", 5),
+            ("set", [(r, c, 0.25) for r, c in far[60:80]] + [(r, c, -0.0) for r, c in far[80:100]]),
+            ("run", 37)]
+    print("case full-grid tiles 160x224 host edits between runs", flush=True)
+    want, _, _ = run_plan(z, h, 0.5, 0.001, plan, {"CALHIP_SCIDDICAT_NO_TILES": "1"})
+    got, launches, _ = run_plan(z, h, 0.5, 0.001, plan, ALWAYS)
+    assert same(got, want)
+    assert np.signbit(got).sum() == np.signbit(want).sum()
+    # without edits: the same run equals the oracle
+    _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 113, False)
+    gh, _, _ = run_plan(z, h, 0.5, 0.001, [("run", 76), ("step", 37)], ALWAYS)
+    assert same(gh, oh)
