@@ -7,17 +7,19 @@ transition, the model's steering — the reference flow reset, declared with
 calhipBindSciddicaTSteering2D so that its device twin runs — and the implicit
 update), with the model already resident in HBM.  Between two steps the
 flows are reset, so a step is a function of z and h alone and the engine
-advances the grid up to nine steps per launch: the tiles within reach of a
-wet cell in LDS, every other cell by a streaming copy, each launch reading
-and writing every cell once.  The same workload with one calRunCAStep2D
-call per step (which the engine groups nine to a launch) is reported
-beside it (`per_step_driver`).  Unit: giga cell updates per second.
+advances the grid up to four steps per launch: the tiles within reach of a
+wet cell in LDS; every other tile stays as it is (h + 0.0), which needs no
+work once both layers hold it (two consecutive copies).  The same workload
+with one calRunCAStep2D call per step (which the engine groups four to a
+launch) is reported beside it (`per_step_driver`).  Unit: giga cell
+updates per second.
 
 The same JSON line carries
-  roofline      the full-grid kernel's achieved algorithmic bandwidth (16 B
-                per cell and launch: h read, h written; z is read only in the
-                tiles near wet cells) from HIP events on the model's stream,
-                against the 8 TB/s HBM peak, and its measured HBM traffic;
+  roofline      the full-grid kernel's measured HBM bytes per launch (the
+                committed rocprofv3 counter summary) over its average launch
+                time (HIP events on the model's stream), against the 8 TB/s
+                HBM peak, beside the launch's minimal bytes (the listed
+                tiles' z and h read, h written);
   cpu_baseline  the reference OpenCAL-OMP build (oracle/_ref, when present;
                 else the oracle restatement) timed on this host's cores on a
                 bounded sample of the same workload;
@@ -59,17 +61,45 @@ MIN_BYTES_PER_CELL_SCIDDICAT = 16.0
 TILES_PROFILE = os.path.join(ROOT, "profiles", "sciddicat_tiles_4096x4096.md")
 
 
-def profiled_traffic(path=TILES_PROFILE):
-    """HBM bytes per launch of the sweep from the committed rocprofv3 counter
-    summary (FETCH_SIZE and WRITE_SIZE, calibrated against a device copy);
+def profiled_traffic(path=TILES_PROFILE, key="traffic_bytes_per_launch"):
+    """Bytes per launch of the full-grid tile kernel from the committed
+    rocprofv3 counter summary: key traffic_bytes_per_launch (FETCH_SIZE and
+    WRITE_SIZE, calibrated against a device copy) or
+    algorithmic_bytes_per_launch (the listed tiles' z and h read, h written);
     None when the summary is absent."""
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return int(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def tiles_roofline(r, size):
+    """The roofline record of the full-grid tile kernel: the measured HBM
+    bytes of a launch (committed counter summary, 4096^2 only) over the
+    launch's average duration in this run, against the HBM peak; the
+    launch's minimal bytes over the same time beside it."""
+    kus = r["kernel_ms_avg"] * 1e3
+    traffic = profiled_traffic() if size == 4096 else None
+    alg = profiled_traffic(key="algorithmic_bytes_per_launch") if size == 4096 else None
+    rl = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+          "traffic": traffic,
+          "traffic_source": "profiles/sciddicat_tiles_4096x4096.md (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE, "
+                            "per 4-step launch, the launches after each model's first two)",
+          "kernel": "plain_tiles<4>", "kernel_us": round(kus, 2), "steps_per_launch": round(r["steps_per_launch"], 4),
+          "algorithmic_bytes_per_launch": alg,
+          "note": "achieved = measured HBM bytes per launch / the launch's average duration (HIP events on the "
+                  "model's stream around the timed steps, over their launches); algorithmic_gbs = the launch's "
+                  "minimal bytes (the listed tiles' z and h read once, h written once, 24 B per cell; the other "
+                  "tiles need nothing once both layers hold them) over the same time.  The launch is bound by "
+                  "the listed tiles' dependent steps on one CU each, not by HBM."}
+    if traffic:
+        rl["achieved"] = round(traffic / (kus * 1e-6) / 1e9, 1)
+        rl["frac"] = round(rl["achieved"] / HBM_PEAK_GBS, 4)
+        rl["algorithmic_gbs"] = round(alg / (kus * 1e-6) / 1e9, 1)
+    return rl
 
 
 def log(*a):
@@ -109,10 +139,6 @@ def bench_sciddicat_plain(n, steps, warmup, seed=2024, batched=True):
     kavg = ktime_ms / max(launches, 1)
     return {
         "wall_s": wall, "gcups": cells * steps / wall / 1e9, "kernel_ms_avg": kavg, "kernel_launches": launches,
-        # the bytes one launch must move at least (h read once and written
-        # once: 16 B per cell, whatever its steps) over the average launch
-        # time (the timed region's GPU time over its launches)
-        "achieved_gbs": MIN_BYTES_PER_CELL_SCIDDICAT * cells / (kavg * 1e-3) / 1e9,
         "steps_per_launch": steps / max(launches, 1),
         "launches_per_step": st["launches"] / max(st["steps"], 1), "mass_rel_err": mass_err,
         "uploads": st["uploads"],
@@ -592,41 +618,14 @@ def main():
         "config": {"workload": f"sciddicaT_plain_{args.size}x{args.size}_fp64", "rows": args.size,
                    "cols": args.size, "p_r": 0.5, "p_epsilon": 0.001, "optimization": "CAL_NO_OPT",
                    "driver": "calRun2D's step loop (calhipRunSteps2D), steering = the reference flow reset "
-                             "with its device twin: up to nine steps per launch"},
-        "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic() if args.size == 4096 else None,
-                     "traffic_source": "profiles/sciddicat_tiles_4096x4096.md (rocprofv3 FETCH_SIZE x 2.0 + "
-                                       "WRITE_SIZE, per 9-step launch)",
-                     "kernel": "plain_tiles<9>", "kernel_us": round(r["kernel_ms_avg"] * 1e3, 2),
-                     "steps_per_launch": round(r["steps_per_launch"], 4),
-                     "algorithmic_bytes_per_launch": int(MIN_BYTES_PER_CELL_SCIDDICAT * args.size * args.size),
-                     "note": "achieved = the bytes one launch must move (h read once and written once: 16 B "
-                             "per cell, whatever its steps; the launch advances 8 steps, the tiles within "
-                             "reach of a wet cell in LDS, reading their z and a halo, every other cell by a "
-                             "copy) / the launch's average duration (HIP events on the model's stream around "
-                             "the timed steps, over their launches); "
-                             "traffic = measured HBM bytes per launch (rocprofv3 counters), traffic_gbs / "
-                             "traffic_frac over the same time"},
+                             "with its device twin: up to four steps per launch"},
+        "roofline": tiles_roofline(r, args.size),
         "per_step_driver": {
-            "driver": "calRunCAStep2D per step (the engine groups nine steps to a launch)",
+            "driver": "calRunCAStep2D per step (the engine groups four steps to a launch)",
             "gcups": round(r1["gcups"], 3),
             "ms_per_step": round(r1["wall_s"] / args.steps * 1e3, 4),
-            "roofline": {"bound": "hbm", "achieved": round(r1["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(r1["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                         "traffic": profiled_traffic() if args.size == 4096 else None,
-                         "traffic_source": "profiles/sciddicat_tiles_4096x4096.md",
-                         "kernel": "plain_tiles<9>", "kernel_us": round(r1["kernel_ms_avg"] * 1e3, 2),
-                         "steps_per_launch": round(r1["steps_per_launch"], 4),
-                         "algorithmic_bytes_per_launch": int(MIN_BYTES_PER_CELL_SCIDDICAT * args.size * args.size)}},
+            "roofline": tiles_roofline(r1, args.size)},
     }
-    # the measured HBM bytes per launch over the same kernel time: the rate the
-    # memory actually sustains, beside the algorithmic rate of `achieved`
-    # (traffic above the algorithmic bytes = re-read halo rows)
-    for rl in (out["roofline"], out["per_step_driver"]["roofline"]):
-        if rl["traffic"]:
-            rl["traffic_gbs"] = round(rl["traffic"] / (rl["kernel_us"] * 1e-6) / 1e9, 1)
-            rl["traffic_frac"] = round(rl["traffic_gbs"] / HBM_PEAK_GBS, 4)
     log("pcie-inclusive run ...")
     out["pcie_inclusive"] = pcie_inclusive_sciddicat(args.size, args.steps)
     if not args.no_cpu:
