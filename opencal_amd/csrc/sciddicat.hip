@@ -890,8 +890,8 @@ struct SciddicaTProgram : Program {
 			const int Kn = pl_pick(n - done > K ? std::min(n - done - K, pl.km) : pl.km);
 			actf::PlainLaunch p = pl_args(m);
 			p.km = Kn;
-			// compute blocks: as many as fit beside each other (two per CU: up
-			// to 1024 threads and 40 KiB of LDS each)
+			// compute blocks: two per CU (576 threads and 23 KiB of LDS each
+			// at K = 4), each taking quarter tiles of the list
 			p.grid_comp = std::max(1, std::min(pl.n_tiles, pl.cus * 2));
 			hipEvent_t ev;
 			timer_begin(m, &ev);

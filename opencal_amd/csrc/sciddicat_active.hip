@@ -313,11 +313,14 @@ __global__ __launch_bounds__(256) void act_normalise(double* __restrict__ h, con
 
 // ---- full grid ----
 
-// Cells per thread of the K-step full-grid kernel: the (16 + 4K)^2 region
-// over at most 1024 threads.
+// The K-step full-grid kernel computes a listed tile in NSUB parts of SUB x
+// SUB cells, one block each, so that a tile's steps spread over several
+// CUs; a part's region with its halo is (SUB + 4K)^2 cells, over at most
+// 1024 threads.
 template <int K>
 struct PlainShape {
-	static constexpr int RW = TILE + 4 * K, NC = RW * RW;
+	static constexpr int SUB = 8, PER_ROW = TILE / SUB, NSUB = PER_ROW * PER_ROW;
+	static constexpr int RW = SUB + 4 * K, NC = RW * RW;
 	static constexpr int CPT = (NC + 1023) / 1024;
 	static constexpr int NT = (NC + CPT - 1) / CPT;
 };
@@ -348,7 +351,9 @@ __global__ __launch_bounds__(TILE * TILE) void plain_setup(PlainLaunch p)
 	}
 }
 
-// Blocks [0, grid_comp): K steps of the listed tiles.  The others: the
+// Blocks [0, grid_comp): K steps of the listed tiles, a block per part of
+// SUB x SUB cells (with its own halo; parts of one tile overlap only in
+// what they read).  The others: the
 // copy h + 0.0 of every unlisted tile, each block a band of 16 rows (one
 // tile row) and 2 * NT columns, two cells per thread and row.
 //
@@ -418,9 +423,14 @@ __global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
 	if (blockIdx.x == 0 && tid == 0) *p.cnt_clear = 0;
 	const int n_in = *p.cnt_in;
 	const double r = p.r, eps = p.eps;
-	for (int k = blockIdx.x; k < n_in; k += p.grid_comp) {
-		const int t = p.list_in[k];
+	for (int k = blockIdx.x; k < n_in * S::NSUB; k += p.grid_comp) {
+		const int t = p.list_in[k / S::NSUB], part = k % S::NSUB;
 		const int ti = t % p.ty, tj = t / p.ty;
+		const int tre = min(TILE, p.rows - ti * TILE), tce = min(TILE, p.cols - tj * TILE);
+		// the part's first row and column in the tile; a part outside the
+		// grid (a ragged last tile) has nothing to do (uniform over the block)
+		const int oa = (part / S::PER_ROW) * S::SUB, ob = (part % S::PER_ROW) * S::SUB;
+		if (oa >= tre || ob >= tce) continue;
 		if (tid == 0) sh_mark = 0u;
 		double z[CPT], h[CPT];
 #pragma unroll
@@ -430,7 +440,8 @@ __global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
 			z[u] = 0.0;
 			h[u] = 0.0;
 			if (q < NC) {
-				const size_t c = (size_t)wrapm(ti * TILE - H + a, p.rows) * p.cols + wrapm(tj * TILE - H + b, p.cols);
+				const size_t c = (size_t)wrapm(ti * TILE + oa - H + a, p.rows) * p.cols +
+				                 wrapm(tj * TILE + ob - H + b, p.cols);
 				h[u] = p.h_in[c];
 				z[u] = p.z[c];
 			}
@@ -479,13 +490,13 @@ __global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
 			// the next step writes su and so only after its first barrier,
 			// when every read of this step is done
 		}
-		const int tre = min(TILE, p.rows - ti * TILE), tce = min(TILE, p.cols - tj * TILE);
 		unsigned mk = 0u;
 #pragma unroll
 		for (int u = 0; u < CPT; ++u) {
 			const int q = tid + u * NT;
-			const int la = q / RW - H, lb = q - (q / RW) * RW - H;
-			if (q < NC && la >= 0 && la < tre && lb >= 0 && lb < tce) {
+			const int a0 = q / RW - H, b0 = q - (q / RW) * RW - H;
+			const int la = oa + a0, lb = ob + b0;
+			if (q < NC && a0 >= 0 && a0 < S::SUB && b0 >= 0 && b0 < S::SUB && la < tre && lb < tce) {
 				const size_t c = (size_t)(ti * TILE + la) * p.cols + tj * TILE + lb;
 				p.h_out[c] = h[u];
 				if (!(h[u] <= eps)) mk |= reach_marks(la, lb, tre, tce, p.km);
