@@ -506,6 +506,9 @@ struct SciddicaTProgram : Program {
 		int ty = 0, tx = 0, n_tiles = 0, km = 0, cus = 0;
 	} pl;
 	bool pl_valid = false;       // pl.list[pl_L % 2] lists the tiles the next launch needs
+	// CALHIP_SCIDDICAT_TILE_PARTS: 1 always quarter tiles, 0 never, unset: by
+	// the list's length
+	int pl_parts = getenv("CALHIP_SCIDDICAT_TILE_PARTS") ? (atoi(getenv("CALHIP_SCIDDICAT_TILE_PARTS")) != 0) : -1;
 	bool pl_declined = false;    // too many tiles near wet cells for the mode to pay
 	int pl_L = 0;                // launch index since the list was built
 	unsigned pl_e = 0;           // tag of the current list
@@ -890,12 +893,18 @@ struct SciddicaTProgram : Program {
 			const int Kn = pl_pick(n - done > K ? std::min(n - done - K, pl.km) : pl.km);
 			actf::PlainLaunch p = pl_args(m);
 			p.km = Kn;
-			// compute blocks: two per CU (576 threads and 23 KiB of LDS each
-			// at K = 4), each taking quarter tiles of the list
+			// compute blocks: two per CU (at K = 4: 1024 threads and 40 KiB of
+			// LDS per whole tile, 576 threads and 23 KiB per quarter tile).
+			// Quarter tiles while the list (its last count read back) leaves
+			// a CU for each part: four blocks per tile then shorten the
+			// launch; a longer list would pay for their halos without
+			// gaining CUs.
 			p.grid_comp = std::max(1, std::min(pl.n_tiles, pl.cus * 2));
+			const int tpl = (actf::TILE / actf::PLAIN_PART) * (actf::TILE / actf::PLAIN_PART);
+			const bool parts = pl_parts >= 0 ? pl_parts != 0 : *pl.h_cnt * tpl <= pl.cus;
 			hipEvent_t ev;
 			timer_begin(m, &ev);
-			if (actf::launch_plain(m->stream, K, p)) return -1;
+			if (actf::launch_plain(m->stream, K, parts, p)) return -1;
 			timer_end(m, ev);
 			m->stats.launches++;
 			flip(h);

@@ -317,9 +317,9 @@ __global__ __launch_bounds__(256) void act_normalise(double* __restrict__ h, con
 // SUB cells, one block each, so that a tile's steps spread over several
 // CUs; a part's region with its halo is (SUB + 4K)^2 cells, over at most
 // 1024 threads.
-template <int K>
+template <int K, int SUB_>
 struct PlainShape {
-	static constexpr int SUB = 8, PER_ROW = TILE / SUB, NSUB = PER_ROW * PER_ROW;
+	static constexpr int SUB = SUB_, PER_ROW = TILE / SUB, NSUB = PER_ROW * PER_ROW;
 	static constexpr int RW = SUB + 4 * K, NC = RW * RW;
 	static constexpr int CPT = (NC + 1023) / 1024;
 	static constexpr int NT = (NC + CPT - 1) / CPT;
@@ -366,10 +366,10 @@ __global__ __launch_bounds__(TILE * TILE) void plain_setup(PlainLaunch p)
 // since, or no copy yet); a launch copies the tile unless 0 < same[t] <=
 // e - 2.  The thread that finds 0 stores e: threads of the same launch that
 // read it afterwards see e and copy as well.
-template <int K>
-__global__ __launch_bounds__(PlainShape<K>::NT) void plain_tiles(PlainLaunch p)
+template <int K, int SUB>
+__global__ __launch_bounds__((PlainShape<K, SUB>::NT)) void plain_tiles(PlainLaunch p)
 {
-	using S = PlainShape<K>;
+	using S = PlainShape<K, SUB>;
 	constexpr int H = 2 * K, RW = S::RW, NC = S::NC, CPT = S::CPT, NT = S::NT;
 	__shared__ double su[NC];
 	__shared__ double so[4][NC];
@@ -561,19 +561,22 @@ int launch_leave(hipStream_t st, double* h, const uint8_t* fl, const uint8_t* gh
 	return CALHIP_CHECK(hipGetLastError());
 }
 
-int plain_threads(int K)
+template <int SUB>
+static int plain_nt(int K)
 {
 	switch (K) {
-	case 1: return PlainShape<1>::NT;
-	case 2: return PlainShape<2>::NT;
-	case 3: return PlainShape<3>::NT;
-	default: return PlainShape<4>::NT;
+	case 1: return PlainShape<1, SUB>::NT;
+	case 2: return PlainShape<2, SUB>::NT;
+	case 3: return PlainShape<3, SUB>::NT;
+	default: return PlainShape<4, SUB>::NT;
 	}
 }
 
-int plain_copy_blocks(int K, int rows, int cols)
+int plain_threads(int K, bool parts) { return parts ? plain_nt<PLAIN_PART>(K) : plain_nt<TILE>(K); }
+
+int plain_copy_blocks(int K, bool parts, int rows, int cols)
 {
-	const int nt = plain_threads(K);
+	const int nt = plain_threads(K, parts);
 	return ((rows + TILE - 1) / TILE) * ((cols + 2 * nt - 1) / (2 * nt));
 }
 
@@ -585,23 +588,26 @@ int launch_plain_setup(hipStream_t st, const PlainLaunch& p)
 }
 
 template <int K>
-static void plain_go(hipStream_t st, int grid, const PlainLaunch& p)
+static void plain_go(hipStream_t st, int grid, bool parts, const PlainLaunch& p)
 {
-	hipLaunchKernelGGL(plain_tiles<K>, dim3(grid), dim3(PlainShape<K>::NT), 0, st, p);
+	if (parts)
+		hipLaunchKernelGGL((plain_tiles<K, PLAIN_PART>), dim3(grid), dim3(PlainShape<K, PLAIN_PART>::NT), 0, st, p);
+	else
+		hipLaunchKernelGGL((plain_tiles<K, TILE>), dim3(grid), dim3(PlainShape<K, TILE>::NT), 0, st, p);
 }
 
-int launch_plain(hipStream_t st, int K, const PlainLaunch& p)
+int launch_plain(hipStream_t st, int K, bool parts, const PlainLaunch& p)
 {
 	if (K < 1 || K > PLAIN_KMAX) {
 		set_error(13, "SciddicaT full-grid launch: steps per launch must be 1 to 4");
 		return -1;
 	}
-	const int grid = p.grid_comp + plain_copy_blocks(K, p.rows, p.cols);
+	const int grid = p.grid_comp + plain_copy_blocks(K, parts, p.rows, p.cols);
 	switch (K) {
-	case 1: plain_go<1>(st, grid, p); break;
-	case 2: plain_go<2>(st, grid, p); break;
-	case 3: plain_go<3>(st, grid, p); break;
-	default: plain_go<4>(st, grid, p); break;
+	case 1: plain_go<1>(st, grid, parts, p); break;
+	case 2: plain_go<2>(st, grid, parts, p); break;
+	case 3: plain_go<3>(st, grid, parts, p); break;
+	default: plain_go<4>(st, grid, parts, p); break;
 	}
 	return CALHIP_CHECK(hipGetLastError());
 }

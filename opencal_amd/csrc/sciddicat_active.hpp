@@ -126,12 +126,16 @@ struct PlainLaunch {
 };
 
 constexpr int PLAIN_KMAX = 4;   // most steps per full-grid launch (1 .. 4)
+constexpr int PLAIN_PART = 8;   // side of a tile part
 
 // threads per block of the K-step kernel and the column width of one copy block
-int plain_threads(int K);
-int plain_copy_blocks(int K, int rows, int cols);
+int plain_threads(int K, bool parts);
+int plain_copy_blocks(int K, bool parts, int rows, int cols);
 int launch_plain_setup(hipStream_t st, const PlainLaunch& p);
-int launch_plain(hipStream_t st, int K, const PlainLaunch& p);
+// parts: each listed tile computed as (TILE / PLAIN_PART)^2 parts of
+// PLAIN_PART x PLAIN_PART cells, one block each (for short lists: a tile's
+// steps spread over several CUs); otherwise one block per tile.
+int launch_plain(hipStream_t st, int K, bool parts, const PlainLaunch& p);
 
 }  // namespace actf
 }  // namespace calhip

@@ -135,3 +135,20 @@ def test_plain_tiles_host_edits_between_launches():
     _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, 113, False)
     gh, _, _ = run_plan(z, h, 0.5, 0.001, [("run", 76), ("step", 37)], ALWAYS)
     assert same(gh, oh)
+
+
+@pytest.mark.parametrize("parts", ["0", "1"])
+@pytest.mark.parametrize("rows,cols,steps,seed", [(64, 64, 61, 1), (37, 53, 45, 2), (17, 17, 40, 4),
+                                                  (130, 260, 90, 6), (3, 3, 12, 8)])
+def test_plain_tiles_whole_or_quarter(parts, rows, cols, steps, seed):
+    """The listed tiles computed whole (one block per tile) or in four 8 x 8
+    parts (one block each), forced either way: both equal the oracle."""
+    print(f"case full-grid tiles parts={parts} {rows}x{cols} steps={steps} seed={seed}", flush=True)
+    z, h, _ = synth.sciddicat_inputs(rows, cols, seed, patch=max(1, min(12, rows // 2)))
+    h[h.shape[0] // 2 + 1, :] = -0.0
+    _, oh = native.oracle_sciddicat(z, h, 0.5, 0.001, steps, False)
+    env = dict(ALWAYS, CALHIP_SCIDDICAT_TILE_PARTS=parts)
+    plans = [[("run", steps)]] + ([[("run", 9), ("step", 5), ("run", steps - 14)]] if steps > 14 else [])
+    for plan in plans:
+        gh, _, _ = run_plan(z, h, 0.5, 0.001, plan, env)
+        assert same(gh, oh), f"plan={plan}"
